@@ -1,0 +1,45 @@
+"""End-to-end golden vector of big-wave (big-wave/test.py:853-960): one point
+source through three gratings at N=2^19, 25 detector pixels, atol 1e-14.
+
+Pins the oracle's propagate, point source, grating slicing, sequencing
+(snapshot-free path, skip_fft final hop), detector, the seeded RNG order of
+the sim-dir setup, the cutoff-angle geometry and the delta/beta restatement."""
+import json
+
+import numpy as np
+
+from oracle import bigwave as bw
+from oracle import xray
+
+
+def snapshot_case(golden_dir):
+    d = json.loads((golden_dir / "bigwave_snapshot.json").read_text())
+    cfg, ms = d["known_config"], d["known_config"]["multisource"]
+    energies, xs = bw.draw_point_sources(ms["seed"], ms["nr_source_points"], ms["energy_range"],
+                                         ms["x_range"])
+    table = []
+    for mat in [("Si", 2.34), ("Au", 19.32)]:
+        de, be = xray.xray_delta_beta(mat[0], mat[1], energies[0])
+        table.append((mat, complex(de, be)))
+    return d, cfg, ms, energies, xs, table
+
+
+def test_deltabeta_restatement_near_subconfig_example(golden_dir):
+    # big-wave/subconfig.yaml:9-12 lists Si/Au deltabeta at 46 keV to 5 digits
+    de, be = xray.xray_delta_beta("Si", 2.34, 46000.0)
+    assert abs(de / 2.2888e-7 - 1) < 1e-4 and abs(be / 2.5847e-10 - 1) < 1e-4
+    de, be = xray.xray_delta_beta("Au", 19.32, 46000.0)
+    assert abs(de / 1.4981e-6 - 1) < 1e-4 and abs(be / 3.6174e-8 - 1) < 1e-4
+
+
+def test_snapshot_golden_vector(golden_dir):
+    d, cfg, ms, energies, xs, table = snapshot_case(golden_dir)
+    p = bw.params_from_config(cfg["sim_params"], bw.convert_energy_wavelength(energies[0]))
+    els = bw.elements_from_config(cfg["elements"], table)
+    angles, _ = bw.compute_cutoff_angles(
+        p.detector_size, p.dx, ms["energy_range"], ms["z"], [e.z_start for e in els],
+        [e.get_thickness() for e in els], p.z_detector,
+        max(abs(ms["x_range"][0]), abs(ms["x_range"][1])))
+    rows = bw.run_simulation(p, bw.PointSource(x=xs[0], z=ms["z"]), els, angles)
+    assert len(rows) == 1
+    assert np.allclose(rows[0], np.array(d["known_data"]), rtol=0, atol=1e-14)
