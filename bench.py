@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark of the split-step hot path on one MI355X.
+
+Workload (default): the Talbot-Lau three-grating chain at 46 keV, one
+simulation per step: plane wave at G0, G0 Au / G1 Si / G2 Au gratings
+(pitch 4.2 um, height 60 um, 64 slices each), 0.2 m start to start, detector
+at the end of G2; N = 2^26 samples of complex64 with FP64 phase arguments,
+dx = 2 nm, 1 um pixels over 1 mm.  One simulation is 194 propagation steps of
+the reference (3 x 64 slices + 2 hops).
+
+Prints one JSON line: sims/hour (value), Gsample-steps/s, HBM roofline of the
+dominant pass kernel (HIP events on the library's stream, algorithmic bytes
+16 B per sample per pass), and the complex128 oracle timed on a bounded
+sample on the host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "rave-sim_amd"))
+
+HBM_PEAK_GBS = 8000.0
+WL_46 = None
+
+
+def cfg2_members(ravesim, N, slices=64, plain=False):
+    wl = ravesim.energy_to_wavelength(46000.0)
+    h, p = 60e-6, 4.2e-6
+    g0 = ravesim.Grating(pitch=p, dc=(0.5, 0.5), z_start=0.0, thickness=h, nr_steps=slices,
+                         db_a=complex(1.1e-6, 4.0e-8))
+    g1 = ravesim.Grating(pitch=p, dc=(0.5, 0.5), z_start=0.2, thickness=h, nr_steps=slices,
+                         db_a=complex(2.6e-7, 1.0e-10))
+    g2 = ravesim.Grating(pitch=p, dc=(0.5, 0.5), z_start=0.4, thickness=h, nr_steps=slices,
+                         db_a=complex(1.1e-6, 4.0e-8))
+    m = ravesim.core.BatchMember(wl=wl, gratings=[g0, g1, g2], cutoff_angles=[math.pi / 2] * 4,
+                                 plane_wave=True, source_z=0.0)
+    return [m], 0.4 + h, 3 * slices + 2
+
+
+def cpu_baseline(N, budget_s=25.0):
+    """Oracle (complex128 numpy/scipy restatement, 1 scipy worker) on whole
+    grating slices of G0 at the benchmark N; scaled to sims/hour."""
+    import numpy as np
+    from oracle import bigwave as bw
+    wl = bw.convert_energy_wavelength(46000.0)
+    dx = 2e-9
+    p = bw.SimParams(N=N, dx=dx, z_detector=0.4 + 60e-6, detector_size=1e-3,
+                     detector_pixel_size_x=1e-6, detector_pixel_size_y=1.0, wl=wl, chunk_size=N)
+    g = bw.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=60e-6 / 64, nr_steps=1,
+                   x_positions=np.array([0.0]), substrate_thickness=0.0, db_a=complex(1.1e-6, 4.0e-8))
+    f = bw.Fields(np.ones(N, dtype=np.complex128))
+    n, t0 = 0, time.perf_counter()
+    while True:
+        bw.grating_apply(f, g, p, math.inf, 0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 8:
+            break
+    per_step = el / n
+    return {"value": 3600.0 / (194 * per_step), "unit": "sims/hour", "cores": 1, "kind": "port",
+            "sample": f"{n} grating slice(s) (mask + FFT, propagator, cutoff, iFFT) of G0 at N=2^{int(math.log2(N))} "
+                      f"complex128, scipy.fft 1 worker, {per_step:.2f} s/step, extrapolated x194 steps per sim"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lgN", type=int, default=26)
+    ap.add_argument("--slices", type=int, default=64)
+    ap.add_argument("--dtype", default="c8", choices=["c8", "c16"])
+    ap.add_argument("--plain", action="store_true", help="one kernel per operation (baseline form)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every configuration of this path runs on a single "
+              "MI355X (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import ravesim
+    from ravesim.core import run_batch
+
+    N = 2 ** args.lgN
+    dtype = np.complex64 if args.dtype == "c8" else np.complex128
+    members, zdet, steps_per_sim = cfg2_members(ravesim, N, args.slices)
+    ctx = ravesim.Context(0)
+    kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet, dtype=dtype, plain=args.plain)
+
+    for _ in range(args.warmup):
+        run_batch(ctx, N, 2e-9, members, **kw)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = run_batch(ctx, N, 2e-9, members, **kw)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    stats = ctx.pass_stats()
+    ctx.set_timing(False)
+    ms_per_step = el * 1e3 / args.steps
+    sims_per_hour = args.steps / el * 3600.0
+    gss = N * steps_per_sim * args.steps / el / 1e9
+
+    dom = max(stats, key=lambda k: stats[k][1])
+    launches, tot_ms, byts = stats[dom]
+    per_launch_ms = tot_ms / max(launches, 1)
+    per_launch_bytes = byts / max(launches, 1)
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tp = Path(args.traffic)
+    if tp.exists():
+        try:
+            traffic = json.loads(tp.read_text()).get(dom)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": f"{dom} pass", "launches": launches, "avg_ms": round(per_launch_ms, 4),
+                "bytes_per_launch": per_launch_bytes,
+                "passes": {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
+                               "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(N)
+
+    line = {
+        "metric": "sims/hour & Gsample-steps/s at 1 GPU (%HBM roofline); p50/p99 latency at fixed QPS",
+        "value": round(sims_per_hour, 2), "unit": "sims/hour", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64+fp64phase" if args.dtype == "c8" else "c128",
+        "data": "synthetic",
+        "config": {"workload": f"Talbot-Lau 3-grating chain, 46 keV, N=2^{args.lgN}, "
+                               f"{args.slices} slices/grating, {steps_per_sim} propagation steps/sim, "
+                               f"{'plain' if args.plain else 'fused'} form",
+                   "N": N, "steps_per_sim": steps_per_sim, "form": "plain" if args.plain else "fused"},
+        "gsample_steps_per_s": round(gss, 3),
+        "hbm_frac_whole_sim": round(gss * 1e9 * (32 if args.dtype == "c8" else 64) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detector_checksum": float(np.sum(out)),
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+/*
+ * ravesim.h -- C-ABI of the MI355X split-step wave-propagation library
+ * (librsim.so).  Plain C types only: pointers, sizes, doubles, status codes.
+ *
+ * The reference has no C-ABI: fast-wave is a standalone executable run once
+ * per source point (fast-wave/main.cpp:18-57) and big-wave drives its Python
+ * path through multisim.run_single_simulation (big-wave/multisim.py:50-150).
+ * Each entry point below names the reference interface it stands in for.
+ *
+ * Error behaviour: every call returns RS_OK (0) or a positive status code and
+ * never exits or throws across the ABI (fast-wave exit(1)s on CUDA errors,
+ * fast-wave/fwcuda/types.hpp:54-67, and throws std::runtime_error on config
+ * errors, fast-wave/src/config_parsing.cpp).  rs_last_error() returns the
+ * message of the calling thread's last failure.
+ *
+ * Threading: one host thread at a time per rs_ctx; distinct contexts may be
+ * used from distinct threads.  All work of a context runs on its own HIP
+ * stream.
+ */
+#ifndef RAVESIM_H
+#define RAVESIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RS_OK 0
+#define RS_ERR_INVALID 1  /* bad argument or handle */
+#define RS_ERR_CONFIG 2   /* malformed or inconsistent sim-dir configuration */
+#define RS_ERR_IO 3       /* file could not be read or written */
+#define RS_ERR_DEVICE 4   /* HIP runtime error */
+#define RS_ERR_NOMEM 5    /* device memory exhausted / admission refused */
+#define RS_ERR_UNSUPPORTED 6
+
+/* field precision: complex64 ("c8") or complex128 ("c16"), big-wave/config.py:171-188 */
+#define RS_C8 8
+#define RS_C16 16
+
+/* execution form */
+#define RS_MODE_FUSED 0 /* fused passes: slice = column pass + row pass */
+#define RS_MODE_PLAIN 1 /* one kernel per operation, fast-wave's launch structure */
+
+typedef struct rs_ctx rs_ctx;
+typedef struct rs_field rs_field;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int rs_version(void);
+
+/* Message of this thread's last failed call ("" if none). */
+const char *rs_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Create a context on HIP device `device`. */
+int rs_ctx_create(int device, rs_ctx **out);
+int rs_ctx_destroy(rs_ctx *ctx);
+/* Block until all work queued on the context's stream has finished. */
+int rs_ctx_synchronize(rs_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* sim-dir entry point                                                 */
+/* ------------------------------------------------------------------ */
+
+typedef struct rs_run_options {
+  int dtype;         /* 0: take `dtype` from config.yaml (default c16, big-wave/config.py:176-177); RS_C8 / RS_C16 force it */
+  int mode;          /* RS_MODE_FUSED or RS_MODE_PLAIN */
+  int write_output;  /* 1: write %08d/detected.npy (float64 (n_steps, n_px)) and u_%04d.npy if requested */
+  int reserved;
+  double *detected;  /* optional caller buffer for the detector rows (n_steps*n_px float64), may be NULL */
+  int64_t detected_capacity; /* elements available in `detected` */
+  int64_t *shape_out;        /* optional int64[2] receiving (n_steps, n_px) */
+} rs_run_options;
+
+/* Run source `source_idx` of the simulation directory `sim_dir`.
+ * Replaces `fastwave <sim_dir> -s <idx>` (fast-wave/main.cpp:18-57 ->
+ * simulation.cpp:375-549) and multisim.run_single_simulation
+ * (big-wave/multisim.py:50-150).  Follows big-wave's semantics where the two
+ * references differ.  `opts` may be NULL (defaults: config dtype, fused,
+ * write_output=1). */
+int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run_options *opts);
+
+/* Parse and validate a sim dir without touching a device (host only).
+ * info receives {N, dtype (8|16), n_steps, n_px, n_elements, n_slices}
+ * where n_slices counts the reference's propagate calls of one phase step.
+ * Same configuration errors as rs_run_simdir. */
+int rs_validate_simdir(const char *sim_dir, int source_idx, int64_t *info, int64_t n_info);
+
+/* ------------------------------------------------------------------ */
+/* field-level operators (fast-wave/fwcuda/wrappers.hpp:10-40,         */
+/* big-wave/propagation.py, optical_element.py, source.py)             */
+/* ------------------------------------------------------------------ */
+
+typedef struct rs_grid {
+  int64_t N;         /* samples; a power of two, 64 <= N <= 2^30 */
+  double dx;         /* sample spacing [m] */
+  int64_t chunk_size;/* big-wave chunk length (grating x is linspace'd per chunk) */
+} rs_grid;
+
+typedef struct rs_grating {
+  double pitch;
+  double dc0, dc1;          /* duty cycle at the first and after the last slice */
+  double thickness;         /* without substrate */
+  int32_t nr_steps;
+  int32_t nr_x_positions;
+  const double *x_positions;
+  double substrate_thickness;
+  double db_a_re, db_a_im;  /* delta + i beta of material A (0 = vacuum) */
+  double db_b_re, db_b_im;
+  double db_s_re, db_s_im;  /* substrate */
+} rs_grating;
+
+typedef struct rs_detector {
+  double size;       /* detector_size [m] */
+  double pixel_x;    /* detector_pixel_size_x */
+  double pixel_y;    /* detector_pixel_size_y */
+} rs_detector;
+
+/* Allocate a field of N samples of precision `dtype` (RS_C8 / RS_C16),
+ * initialised to zero.  Mode selects fused or plain kernels for operators. */
+int rs_field_create(rs_ctx *ctx, const rs_grid *grid, int dtype, int mode, rs_field **out);
+int rs_field_destroy(rs_field *f);
+/* Copy a natural-order complex field from / to the host.  `host` holds N
+ * interleaved complex values of the field's precision. */
+int rs_field_upload(rs_field *f, const void *host);
+int rs_field_download(rs_field *f, void *host);
+
+/* u = exp(-2 pi i r / wl) / sqrt(r), r = sqrt((x - x_source)^2 + z^2):
+ * big-wave/propagation.py:141-151, fast-wave/fwcuda/kernels.cu:6-28. */
+int rs_point_source(rs_field *f, double wl, double x_source, double z);
+
+/* u <- IFFT(FFT(u) * exp(-2 pi i dz/wl) exp(i pi wl dz f^2) * [|f| < f_c]):
+ * big-wave/propagation.py:100-138, fast-wave/src/simulation.cpp:34-40.
+ * The cutoff is applied exactly as big-wave applies it: bins
+ * [ceil(N f_c dx), N - ceil(N f_c dx) + 1) are zeroed unless
+ * ceil(N f_c dx) > N/2; pass INFINITY to keep every frequency. */
+int rs_propagate(rs_field *f, double wl, double dz, double cutoff_freq);
+
+/* Grating slices plus substrate (big-wave/optical_element.py:134-206,
+ * fast-wave/src/simulation.cpp:131-150), using x_positions[phase_step]. */
+int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step,
+                     double cutoff_freq);
+
+/* u *= factor (fast-wave/fwcuda/kernels.cu:160-172). */
+int rs_scale(rs_field *f, double re, double im);
+
+/* Detector intensity at distance z into out[n_px], n_px = floor(size/pixel_x):
+ * big-wave/propagation.py:279-320 (np.histogram binning, FP64 sums). */
+int rs_square_and_downsample(rs_field *f, const rs_detector *det, double z, double *out,
+                             int64_t n_out);
+
+/* Number of detector pixels for `det` (int(size // pixel_x)). */
+int64_t rs_detector_pixels(const rs_detector *det);
+
+/* Sample-index boundaries of the detector pixels (host-only helper, no
+ * device work): bounds[k]..bounds[k+1] are the samples binned into pixel k.
+ * bounds has n_px + 1 entries. */
+int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bounds,
+                       int64_t n_bounds);
+
+/* ------------------------------------------------------------------ */
+/* batched runner (many energies / sources / sims in lockstep)        */
+/* ------------------------------------------------------------------ */
+
+/* One (energy, source) simulation of a batch.  All members of one batch call
+ * share N, dtype and the element structure; they differ in wavelength,
+ * source position and grating parameters. */
+typedef struct rs_batch_member {
+  double wl;               /* wavelength [m] */
+  double weight;           /* weight of its detector rows in the accumulated sum */
+  int32_t source_kind;     /* 0 = point source, 1 = plane wave (vector source of ones) */
+  int32_t out_index;       /* which accumulator row receives this member */
+  double source_x, source_z;
+  const rs_grating *gratings; /* nr_gratings entries */
+  const double *cutoff_angles; /* nr_gratings + 1 entries */
+} rs_batch_member;
+
+typedef struct rs_batch_desc {
+  rs_grid grid;
+  rs_detector det;
+  double z_detector;
+  int32_t dtype;
+  int32_t mode;
+  int32_t nr_gratings;
+  int32_t nr_members;
+  const rs_batch_member *members;
+  int32_t nr_outputs;      /* accumulator rows */
+  int32_t max_resident;    /* members resident at once (0 = as many as memory allows) */
+  int32_t use_graph;       /* capture each group's launch sequence in a hipGraph */
+  int32_t reserved;
+} rs_batch_desc;
+
+/* Run every member; out[nr_outputs * n_px] receives sum over members of
+ * weight * detector row (FP64).  Only single-phase-step setups. */
+int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *desc, double *out, int64_t n_out);
+
+/* ------------------------------------------------------------------ */
+/* measurement (no counterpart in the reference; fast-wave logs        */
+/* per-element cudaEvent timings, fast-wave/src/simulation.cpp:13-32)  */
+/* ------------------------------------------------------------------ */
+
+/* Per kernel class: 0 row pass, 1 top column pass, 2 middle column pass,
+ * 3 other (plain-form elementwise kernels). */
+typedef struct rs_pass_stat {
+  int64_t launches;
+  double total_ms;   /* summed HIP-event durations on the context stream */
+  double bytes;      /* summed algorithmic bytes (every field read + written once) */
+} rs_pass_stat;
+
+/* Enable (1) / disable (0) per-launch event timing and reset the counters. */
+int rs_set_timing(rs_ctx *ctx, int enable);
+/* Copy min(n, 4) class records. */
+int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAVESIM_H */

@@ -1,0 +1,429 @@
+// C-ABI of librsim.so (declarations and reference citations in include/ravesim.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ravesim.h"
+#include "engine.hpp"
+#include "npy.hpp"
+#include "simulation.hpp"
+
+struct rs_ctx {
+  std::unique_ptr<rs::Context> ctx;
+};
+
+struct rs_field {
+  rs_ctx *owner = nullptr;
+  rs_grid grid{};
+  bool f64 = false;
+  int mode = 0;
+  rs::DevBuf buf;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return RS_OK;
+  } catch (const rs::Error &e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(RS_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(RS_ERR_INVALID, e.what());
+  }
+}
+
+rs::RunConfig field_rc(const rs_field *f) {
+  rs::RunConfig rc;
+  rc.N = f->grid.N;
+  rc.dx = f->grid.dx;
+  rc.f64 = f->f64;
+  rc.mode = f->mode;
+  rc.n_out = 1;
+  return rc;
+}
+
+void run_on_field(rs_field *f, rs::Member &m, rs::RunConfig rc, double *out = nullptr, int64_t n_out = 0) {
+  void *fields[1] = {f->buf.p};
+  rc.fields = fields;
+  std::vector<rs::Member> ms;
+  ms.push_back(std::move(m));
+  std::vector<double> det = rs::run_members(*f->owner->ctx, rc, ms);
+  if (out) {
+    if ((int64_t)det.size() > n_out) throw rs::Error(RS_ERR_INVALID, "output buffer too small");
+    std::memcpy(out, det.data(), det.size() * sizeof(double));
+  }
+}
+
+rs::SimConfig field_cfg(const rs_field *f, double wl) {
+  rs::SimConfig c;
+  c.N = f->grid.N;
+  c.dx = f->grid.dx;
+  c.chunk_size = f->grid.chunk_size > 0 ? f->grid.chunk_size : f->grid.N;
+  c.f64 = f->f64;
+  c.wl = wl;
+  return c;
+}
+
+rs::GratingCfg grating_from(const rs_grating *g) {
+  rs::GratingCfg c;
+  c.pitch = g->pitch;
+  c.dc0 = g->dc0;
+  c.dc1 = g->dc1;
+  c.thickness = g->thickness;
+  c.nr_steps = g->nr_steps;
+  c.x_positions.assign(g->x_positions, g->x_positions + g->nr_x_positions);
+  c.substrate_thickness = g->substrate_thickness;
+  c.db_a = {g->db_a_re, g->db_a_im};
+  c.db_b = {g->db_b_re, g->db_b_im};
+  c.db_s = {g->db_s_re, g->db_s_im};
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_version(void) { return 100; }
+
+const char *rs_last_error(void) { return g_err.c_str(); }
+
+int rs_ctx_create(int device, rs_ctx **out) {
+  if (!out) return fail(RS_ERR_INVALID, "out is NULL");
+  return guarded([&] {
+    int n = 0;
+    rs::hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw rs::Error(RS_ERR_INVALID, "no HIP device " + std::to_string(device));
+    auto c = std::make_unique<rs_ctx>();
+    c->ctx = std::make_unique<rs::Context>(device);
+    *out = c.release();
+  });
+}
+
+int rs_ctx_destroy(rs_ctx *ctx) {
+  if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  return guarded([&] { delete ctx; });
+}
+
+int rs_ctx_synchronize(rs_ctx *ctx) {
+  if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  return guarded([&] { ctx->ctx->sync(); });
+}
+
+int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run_options *opts) {
+  if (!ctx || !sim_dir) return fail(RS_ERR_INVALID, "ctx or sim_dir is NULL");
+  rs_run_options o{};
+  o.write_output = 1;
+  if (opts) o = *opts;
+  return guarded([&] {
+    rs::SimConfig c = rs::parse_simdir(sim_dir, source_idx);
+    if (o.dtype == RS_C8) c.f64 = false;
+    else if (o.dtype == RS_C16) c.f64 = true;
+    else if (o.dtype != 0) throw rs::Error(RS_ERR_INVALID, "dtype must be 0, RS_C8 or RS_C16");
+    const std::string sub = std::string(sim_dir) + "/" + rs::subdir_name(source_idx);
+    rs::Member m;
+    std::vector<std::vector<char>> saved;
+    const bool save_u = c.save_final_u_vectors && o.write_output;
+    if (save_u) m.saved_u = &saved;
+    std::vector<char> src_host;
+    if (c.source.kind == rs::SourceCfg::Vector) {
+      rs::NpyArray a = rs::npy_load(c.source.input_path);
+      if (a.shape.size() != 1 || a.shape[0] != c.N)
+        throw rs::Error(RS_ERR_CONFIG, "source vector must have shape (N,)");
+      const int64_t n = c.N;
+      src_host.resize((size_t)n * (c.f64 ? 16 : 8));
+      if (a.descr == "<c16") {
+        const double *p = (const double *)a.data.data();
+        if (c.f64) std::memcpy(src_host.data(), p, src_host.size());
+        else {
+          float *q = (float *)src_host.data();
+          for (int64_t i = 0; i < 2 * n; ++i) q[i] = (float)p[i];
+        }
+      } else if (a.descr == "<c8") {
+        const float *p = (const float *)a.data.data();
+        if (!c.f64) std::memcpy(src_host.data(), p, src_host.size());
+        else {
+          double *q = (double *)src_host.data();
+          for (int64_t i = 0; i < 2 * n; ++i) q[i] = (double)p[i];
+        }
+      } else {
+        throw rs::Error(RS_ERR_CONFIG, "source vector must be complex64 or complex128");
+      }
+      m.upload = src_host.data();
+    }
+    const int nsteps = rs::build_member(c, m, 0, save_u);
+    rs::RunConfig rc;
+    rc.N = c.N;
+    rc.dx = c.dx;
+    rc.f64 = c.f64;
+    rc.mode = o.mode;
+    rc.det_size = c.detector_size;
+    rc.det_px = c.px;
+    rc.det_py = c.py;
+    rc.z_det = c.z_detector;
+    rc.n_out = nsteps;
+    std::vector<rs::Member> ms;
+    ms.push_back(std::move(m));
+    std::vector<double> det = rs::run_members(*ctx->ctx, rc, ms);
+    const int64_t npx = rs::detector_pixels(c.detector_size, c.px);
+    if (o.shape_out) {
+      o.shape_out[0] = nsteps;
+      o.shape_out[1] = npx;
+    }
+    if (o.detected) {
+      if (o.detected_capacity < (int64_t)det.size()) throw rs::Error(RS_ERR_INVALID, "detected buffer too small");
+      std::memcpy(o.detected, det.data(), det.size() * sizeof(double));
+    }
+    if (o.write_output) {
+      rs::npy_save(sub + "/detected.npy", "<f8", {nsteps, npx}, det.data(), det.size() * sizeof(double));
+      for (size_t s = 0; s < saved.size(); ++s) {
+        char name[32];
+        std::snprintf(name, sizeof name, "/u_%04d.npy", (int)s);
+        rs::npy_save(sub + name, c.f64 ? "<c16" : "<c8", {c.N}, saved[s].data(), saved[s].size());
+      }
+    }
+  });
+}
+
+int rs_validate_simdir(const char *sim_dir, int source_idx, int64_t *info, int64_t n_info) {
+  if (!sim_dir) return fail(RS_ERR_INVALID, "sim_dir is NULL");
+  return guarded([&] {
+    rs::SimConfig c = rs::parse_simdir(sim_dir, source_idx);
+    rs::Member m;
+    const int nsteps = rs::build_member(c, m, 0, false);
+    int64_t props = 0;
+    for (const auto &op : m.ops) {
+      if (op.kind == rs::OpKind::Detect) break;
+      props += op.kind == rs::OpKind::Prop;
+    }
+    const int64_t v[6] = {c.N, c.f64 ? 16 : 8, nsteps, rs::detector_pixels(c.detector_size, c.px),
+                          (int64_t)c.elements.size(), props};
+    if (info)
+      for (int64_t i = 0; i < n_info && i < 6; ++i) info[i] = v[i];
+  });
+}
+
+int rs_field_create(rs_ctx *ctx, const rs_grid *grid, int dtype, int mode, rs_field **out) {
+  if (!ctx || !grid || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  if (mode != RS_MODE_FUSED && mode != RS_MODE_PLAIN) return fail(RS_ERR_INVALID, "unknown mode");
+  if (grid->N < 64 || (grid->N & (grid->N - 1)) != 0 || grid->N > (int64_t(1) << 30))
+    return fail(RS_ERR_UNSUPPORTED, "N must be a power of two in [2^6, 2^30]");
+  if (!(grid->dx > 0)) return fail(RS_ERR_INVALID, "dx must be positive");
+  return guarded([&] {
+    auto f = std::make_unique<rs_field>();
+    f->owner = ctx;
+    f->grid = *grid;
+    if (f->grid.chunk_size <= 0) f->grid.chunk_size = grid->N;
+    f->f64 = dtype == RS_C16;
+    f->mode = mode;
+    rs::hip_check(hipSetDevice(ctx->ctx->device), "hipSetDevice");
+    (void)ctx->ctx->plan(grid->N, f->f64);
+    f->buf = rs::DevBuf((size_t)grid->N * (f->f64 ? 16 : 8));
+    rs::hip_check(hipMemsetAsync(f->buf.p, 0, f->buf.bytes, ctx->ctx->stream), "zero field");
+    rs::hip_check(hipStreamSynchronize(ctx->ctx->stream), "sync");
+    *out = f.release();
+  });
+}
+
+int rs_field_destroy(rs_field *f) {
+  if (!f) return fail(RS_ERR_INVALID, "field is NULL");
+  return guarded([&] { delete f; });
+}
+
+int rs_field_upload(rs_field *f, const void *host) {
+  if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    rs::hip_check(hipMemcpyAsync(f->buf.p, host, f->buf.bytes, hipMemcpyHostToDevice, f->owner->ctx->stream),
+                  "upload");
+    rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
+  });
+}
+
+int rs_field_download(rs_field *f, void *host) {
+  if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    rs::hip_check(hipMemcpyAsync(host, f->buf.p, f->buf.bytes, hipMemcpyDeviceToHost, f->owner->ctx->stream),
+                  "download");
+    rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
+  });
+}
+
+int rs_point_source(rs_field *f, double wl, double x_source, double z) {
+  if (!f) return fail(RS_ERR_INVALID, "field is NULL");
+  if (!(wl > 0)) return fail(RS_ERR_INVALID, "wavelength must be positive");
+  return guarded([&] {
+    rs::Member m;
+    m.wl = wl;
+    rs::Op s{rs::OpKind::Source};
+    s.x.kind = rs::XOP_POINT;
+    s.x.post[0] = 1.0;
+    s.x.xs = x_source;
+    s.x.z = z;
+    s.x.dx = f->grid.dx;
+    s.x.kphase = -2.0 * M_PI / wl;
+    m.ops.push_back(s);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
+int rs_propagate(rs_field *f, double wl, double dz, double cutoff_freq) {
+  if (!f) return fail(RS_ERR_INVALID, "field is NULL");
+  if (!(wl > 0)) return fail(RS_ERR_INVALID, "wavelength must be positive");
+  return guarded([&] {
+    rs::Member m;
+    m.wl = wl;
+    rs::Op p{rs::OpKind::Prop};
+    p.dz = dz;
+    p.fc = cutoff_freq;
+    m.ops.push_back(p);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
+int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step, double cutoff_freq) {
+  if (!f || !g) return fail(RS_ERR_INVALID, "NULL argument");
+  if (g->nr_steps < 1 || g->nr_x_positions < 1 || !g->x_positions)
+    return fail(RS_ERR_INVALID, "grating needs nr_steps >= 1 and at least one x position");
+  if (phase_step < 0 || phase_step >= g->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  return guarded([&] {
+    rs::SimConfig c = field_cfg(f, wl);
+    const rs::GratingCfg gc = grating_from(g);
+    rs::Member m;
+    m.wl = wl;
+    rs::grating_ops(c, gc, phase_step, cutoff_freq, m.ops);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
+int rs_scale(rs_field *f, double re, double im) {
+  if (!f) return fail(RS_ERR_INVALID, "field is NULL");
+  return guarded([&] {
+    rs::Member m;
+    m.wl = 1.0;
+    rs::Op mk{rs::OpKind::Mask};
+    mk.x.kind = rs::XOP_SCALAR;
+    mk.x.fa[0] = re;
+    mk.x.fa[1] = im;
+    mk.x.post[0] = 1.0;
+    m.ops.push_back(mk);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
+int64_t rs_detector_pixels(const rs_detector *det) {
+  if (!det) return -1;
+  return rs::detector_pixels(det->size, det->pixel_x);
+}
+
+int rs_square_and_downsample(rs_field *f, const rs_detector *det, double z, double *out, int64_t n_out) {
+  if (!f || !det || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    const int64_t npx = rs::detector_pixels(det->size, det->pixel_x);
+    if (npx > f->grid.N) throw rs::Error(RS_ERR_INVALID, "detector has more pixels than samples");
+    if (n_out < npx) throw rs::Error(RS_ERR_INVALID, "output buffer too small");
+    rs::Member m;
+    m.wl = 1.0;
+    rs::Op d{rs::OpKind::Detect};
+    d.row = 0;
+    m.ops.push_back(d);
+    rs::RunConfig rc = field_rc(f);
+    rc.det_size = det->size;
+    rc.det_px = det->pixel_x;
+    rc.det_py = det->pixel_y;
+    rc.z_det = z;
+    run_on_field(f, m, rc, out, n_out);
+  });
+}
+
+int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bounds, int64_t n_bounds) {
+  if (!det || !bounds) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    std::vector<int64_t> b = rs::detector_bounds(N, dx, det->size, det->pixel_x);
+    if ((int64_t)b.size() > n_bounds) throw rs::Error(RS_ERR_INVALID, "bounds buffer too small");
+    std::memcpy(bounds, b.data(), b.size() * sizeof(int64_t));
+  });
+}
+
+int rs_set_timing(rs_ctx *ctx, int enable) {
+  if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  ctx->ctx->timing = enable != 0;
+  for (auto &s : ctx->ctx->stats) s = rs::PassStat{};
+  return RS_OK;
+}
+
+int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n) {
+  if (!ctx || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  for (int i = 0; i < n && i < rs::PC_COUNT; ++i) {
+    out[i].launches = ctx->ctx->stats[i].launches;
+    out[i].total_ms = ctx->ctx->stats[i].ms;
+    out[i].bytes = ctx->ctx->stats[i].bytes;
+  }
+  return RS_OK;
+}
+
+int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out) {
+  if (!ctx || !d || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (d->nr_members < 1 || !d->members) return fail(RS_ERR_INVALID, "batch has no members");
+  if (d->dtype != RS_C8 && d->dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  return guarded([&] {
+    const int64_t npx = rs::detector_pixels(d->det.size, d->det.pixel_x);
+    if (n_out < (int64_t)d->nr_outputs * npx) throw rs::Error(RS_ERR_INVALID, "output buffer too small");
+    std::vector<rs::Member> ms(d->nr_members);
+    for (int i = 0; i < d->nr_members; ++i) {
+      const rs_batch_member &bm = d->members[i];
+      if (bm.out_index < 0 || bm.out_index >= d->nr_outputs) throw rs::Error(RS_ERR_INVALID, "out_index out of range");
+      rs::SimConfig c;
+      c.N = d->grid.N;
+      c.dx = d->grid.dx;
+      c.chunk_size = d->grid.chunk_size > 0 ? d->grid.chunk_size : d->grid.N;
+      c.f64 = d->dtype == RS_C16;
+      c.z_detector = d->z_detector;
+      c.detector_size = d->det.size;
+      c.px = d->det.pixel_x;
+      c.py = d->det.pixel_y;
+      c.wl = bm.wl;
+      c.source.kind = bm.source_kind == 1 ? rs::SourceCfg::Plane : rs::SourceCfg::Point;
+      c.source.x = bm.source_x;
+      c.source.z = bm.source_z;
+      for (int k = 0; k < d->nr_gratings; ++k) c.elements.push_back(grating_from(&bm.gratings[k]));
+      for (int k = 0; k <= d->nr_gratings; ++k) c.cutoff_angles.push_back(bm.cutoff_angles[k]);
+      const int nsteps = rs::build_member(c, ms[i], bm.out_index, false);
+      if (nsteps != 1) throw rs::Error(RS_ERR_UNSUPPORTED, "batched members must not be phase stepped");
+      ms[i].weight = bm.weight;
+    }
+    rs::RunConfig rc;
+    rc.N = d->grid.N;
+    rc.dx = d->grid.dx;
+    rc.f64 = d->dtype == RS_C16;
+    rc.mode = d->mode;
+    rc.det_size = d->det.size;
+    rc.det_px = d->det.pixel_x;
+    rc.det_py = d->det.pixel_y;
+    rc.z_det = d->z_detector;
+    rc.n_out = d->nr_outputs;
+    rc.max_resident = d->max_resident;
+    rc.use_graph = d->use_graph != 0;
+    std::vector<double> det = rs::run_members(*ctx->ctx, rc, ms);
+    std::memcpy(out, det.data(), det.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"

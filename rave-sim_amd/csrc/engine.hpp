@@ -1,0 +1,165 @@
+// Host engine: FFT plans, member programs and the lockstep executor.
+//
+// A *member* is one (source point, energy) simulation, the unit fast-wave runs
+// per process (fast-wave/src/simulation.cpp:375-549).  Its work is a list of
+// Ops mirroring big-wave's driver (big-wave/wavesim.py:84-298): generate or
+// upload the source, x-space multiplies (grating slices, substrates),
+// propagations, detector reads, snapshot / restore for phase stepping.
+// Members whose op lists have the same shape run in lockstep: every fused
+// pass is one launch over all of them (blockIdx.y = member).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rs_device.hpp"
+
+namespace rs {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+
+// ----------------------------------------------------------------------------
+// exact host helpers (big-wave arithmetic)
+// ----------------------------------------------------------------------------
+
+// [rs, re) zeroed by the cutoff, big-wave/propagation.py:83-90; (0,0) if none
+void cutoff_range(int64_t N, double fc, double dx, int64_t *rs, int64_t *re);
+
+// number of detector pixels, int(size // px) (big-wave/propagation.py:286)
+int64_t detector_pixels(double size, double px);
+
+// Sample boundaries per pixel reproducing np.histogram's uniform-bin
+// assignment of x_j = (j - N/2) dx over [-(n/2) px, (n/2 + 1) px) with n+1
+// bins (big-wave/propagation.py:293-308).  bounds has n_px + 1 entries.
+std::vector<int64_t> detector_bounds(int64_t N, double dx, double size, double px);
+
+// final per-pixel factor cos(atan(x/z)) dx py / sqrt(x^2 + z^2),
+// x = (k - n//2) px (big-wave/propagation.py:312-320, util.py:78-82)
+std::vector<double> detector_scale(double size, double px, double py, double dx, double z);
+
+// exp(2 pi i t / wl * db), big-wave/optical_element.py:36-39
+void material_factor(double db_re, double db_im, double t, double wl, double out[2]);
+
+double energy_to_wavelength(double e);
+
+// ----------------------------------------------------------------------------
+// FFT plan
+// ----------------------------------------------------------------------------
+
+struct Plan {
+  int64_t N = 0;
+  int lgN = 0;
+  bool f64 = false;
+  int ngroups = 0;
+  int lgA = 0, lgB = 0, lgR = 0;  // top, middle, row (lgA = lgR when ngroups == 1)
+  // device tables (owned)
+  void *fib[3] = {nullptr, nullptr, nullptr};  // per pass: W_L^j
+  void *hi[2] = {nullptr, nullptr}, *lo[2] = {nullptr, nullptr};
+  int sh[2] = {0, 0};
+  Plan() = default;
+  Plan(const Plan &) = delete;
+  Plan &operator=(const Plan &) = delete;
+  ~Plan();
+  Geom geom_row() const;
+  Geom geom_top() const;
+  Geom geom_mid() const;
+  std::string describe() const;
+};
+
+// default split of lgN into digit groups (override with lgR >= 0)
+void plan_split(int lgN, bool f64, int lgR_req, int *ng, int *lgA, int *lgB, int *lgR);
+std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req = -1);
+
+// ----------------------------------------------------------------------------
+// member programs
+// ----------------------------------------------------------------------------
+
+enum class OpKind { Source, Upload, Mask, Prop, Detect, Snapshot, Restore, SaveU, Flush };
+
+struct Op {
+  OpKind kind;
+  XOp x{};          // Source (POINT / ONES) and Mask (MASK / SCALAR)
+  double dz = 0;    // Prop
+  double fc = 0;    // Prop cutoff frequency (big-wave semantics)
+  int row = 0;      // Detect: output row; SaveU: phase step
+};
+
+struct Member {
+  double wl = 0;
+  double weight = 1.0;
+  int out_base = 0;             // accumulator row of Detect(row = 0)
+  std::vector<Op> ops;
+  const void *upload = nullptr; // host data for the Upload op (natural order)
+  std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
+};
+
+struct RunConfig {
+  int64_t N = 0;
+  double dx = 0;
+  bool f64 = false;
+  int mode = 0;              // 0 fused, 1 plain
+  double det_size = 0, det_px = 0, det_py = 1, z_det = 0;
+  int n_out = 1;             // accumulator rows
+  int max_resident = 0;      // 0 = all members at once
+  bool use_graph = false;
+  int lgR = -1;              // row length override (autotuner)
+  void *const *fields = nullptr;  // caller-owned field buffers (one per member), else allocated
+};
+
+class Context;
+
+// Run all members; returns scaled detector rows, n_out x n_px (row-major).
+std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<Member> &members);
+
+// ----------------------------------------------------------------------------
+// context: device + stream + plan cache
+// ----------------------------------------------------------------------------
+
+// per-kernel-class timing collected with HIP events on the context stream
+struct PassStat {
+  int64_t launches = 0;
+  double ms = 0;      // summed kernel durations
+  double bytes = 0;   // summed algorithmic bytes (fields read + written)
+};
+enum PassClass { PC_ROW = 0, PC_TOP = 1, PC_MID = 2, PC_OTHER = 3, PC_COUNT = 4 };
+
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+  int device;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  PassStat stats[PC_COUNT];
+  Plan &plan(int64_t N, bool f64, int lgR = -1);
+  void sync();
+
+ private:
+  std::vector<std::unique_ptr<Plan>> plans_;
+  std::vector<int> plan_lgR_;
+};
+
+// device buffer RAII
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b);
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept;
+  ~DevBuf();
+};
+
+}  // namespace rs

@@ -1,0 +1,82 @@
+// Structures shared between the host scheduler and the gfx950 kernels.
+//
+// A field of N = 2^n complex samples is transformed with a digit-group
+// (four-step) FFT.  The index bits are split into 1..3 groups: the lowest
+// group is the contiguous "row" of length Lr, the higher groups are strided
+// "columns".  One split-step propagation is
+//   [top column pass: inverse col DFT of the previous slice, x-space multiply
+//    (grating / substrate / source generation), forward col DFT + twiddle]
+//   [middle column pass, forward]            (3 groups only)
+//   [row pass: row DFT, spectrum multiply exp(i phi(k)) * cutoff * 1/N, row iDFT]
+//   [middle column pass, inverse]            (3 groups only)
+// so that the field never returns to natural order between slices.
+#pragma once
+#include <stdint.h>
+
+namespace rs {
+
+enum XopKind : int32_t {
+  XOP_NONE = 0,
+  XOP_MASK = 1,    // grating: factor = frac(x/pitch + dc/2) < dc ? fa : fb
+  XOP_SCALAR = 2,  // u *= fa
+  XOP_POINT = 3,   // u = exp(i kphase r) / sqrt(r)   (generated, input not read)
+  XOP_ONES = 4,    // u = 1 (plane wave, generated)
+};
+
+// x-space (natural order) elementwise operation.  Every kind is followed by a
+// multiplication with `post` (substrate factors and other scalars fold here).
+struct XOp {
+  int32_t kind;
+  int32_t single_chunk;  // 1 when N <= chunk (grating x needs no chunk index)
+  double fa[2], fb[2];
+  double post[2];
+  // grating (big-wave/optical_element.py:341-357)
+  double pitch, dc, half_dc, xpos, dx;
+  int64_t chunk;
+  // point source (big-wave/propagation.py:141-151)
+  double xs, z, kphase;
+};
+
+// spectrum multiply in the row pass: exp(2 pi i (alpha k'^2 + beta)) * scale,
+// zero for k in [rs, re) (big-wave/propagation.py:74-138).
+struct KOp {
+  int32_t on;
+  int32_t pad_;
+  double alpha_rev, beta_rev;
+  int64_t rs, re;
+  double scale[2];
+};
+
+struct FieldArgs {
+  void *data;      // field base pointer (float2* or double2*)
+  int32_t do_inv;  // column: conj twiddle + inverse DFT first; row: inverse DFT last
+  int32_t do_fwd;  // forward DFT (+ twiddle for columns)
+  XOp x;
+  KOp k;
+};
+
+// detector launch record (one per field)
+struct DetArgs {
+  const void *data;
+  double *acc;      // n_px accumulators of this member's output row
+  double weight;
+  int32_t accumulate;
+  int32_t pad_;
+};
+
+// geometry of one pass over a plan
+struct Geom {
+  int64_t N;
+  int64_t S;       // element stride of this pass's fibers (1 for the row pass)
+  int32_t top;     // this pass holds the highest digit group (x-space ops allowed)
+  int32_t ngroups;
+  int32_t lgA, lgB, lgR;  // log2 of top / middle / row group sizes (lgB = 0 if 2 groups)
+  // inter-group twiddle W_M^e, e = k*lo, M = L*S, split as hi[e >> sh] * lo[e & mask]
+  int32_t tw_sh;
+  int32_t pad_;
+  const void *tw_hi;
+  const void *tw_lo;
+  const void *tw_fib;  // W_L^j, j < L
+};
+
+}  // namespace rs

@@ -1,0 +1,348 @@
+#include "simulation.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+#include "yaml_lite.hpp"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace rs {
+
+std::string subdir_name(int source_idx) {
+  char b[32];
+  std::snprintf(b, sizeof b, "%08d", source_idx);
+  return b;
+}
+
+namespace {
+
+double get_f(const YNode &n, const std::string &k) { return n[k].as_double(k); }
+
+struct DbEntry {
+  std::string name;
+  double density;
+  std::complex<double> db;
+};
+
+// closest-density match within 1e-5 (big-wave/optical_element.py:42-63)
+std::complex<double> lookup(const std::vector<DbEntry> &t, const YNode &mat, const std::string &what) {
+  if (mat.is_null()) return {0, 0};
+  if (mat.type != YNode::Seq || mat.size() < 2) throw Error(2, "material " + what + " must be [name, density]");
+  const std::string name = mat[0].as_string(what);
+  const double dens = mat[1].as_double(what + ".density");
+  const DbEntry *best = nullptr;
+  double bd = 0;
+  for (auto &e : t) {
+    if (e.name != name) continue;
+    const double d = std::fabs(e.density - dens);
+    if (!best || d < bd) {
+      best = &e;
+      bd = d;
+    }
+  }
+  if (!best) throw Error(2, "Material " + name + " not found in deltabeta table");
+  if (!(bd < 1e-5)) throw Error(2, "Material " + name + " found with density error of " + std::to_string(bd));
+  return best->db;
+}
+
+}  // namespace
+
+SimConfig parse_simdir(const std::string &sim_dir, int source_idx) {
+  const YNode cfg = yaml_load_file(sim_dir + "/config.yaml");
+  const YNode comp = yaml_load_file(sim_dir + "/computed.yaml");
+  const YNode sub = yaml_load_file(sim_dir + "/" + subdir_name(source_idx) + "/subconfig.yaml");
+  SimConfig c;
+  const YNode &sp = cfg["sim_params"];
+  c.N = sp["N"].as_int("N");
+  c.dx = get_f(sp, "dx");
+  c.z_detector = get_f(sp, "z_detector");
+  c.detector_size = get_f(sp, "detector_size");
+  c.px = get_f(sp, "detector_pixel_size_x");
+  c.py = get_f(sp, "detector_pixel_size_y");
+  c.chunk_size = sp["chunk_size"].as_int("chunk_size");
+  if (cfg.has("dtype") && !cfg["dtype"].is_null()) {
+    const std::string d = cfg["dtype"].as_string("dtype");
+    if (d == "c8") c.f64 = false;
+    else if (d == "c16") c.f64 = true;
+    else throw Error(2, "Unknown dtype: " + d + ". Has to be either c8 or c16");
+  }
+  c.save_final_u_vectors = cfg["save_final_u_vectors"].as_bool("save_final_u_vectors");
+  for (size_t i = 0; i < comp["cutoff_angles"].size(); ++i)
+    c.cutoff_angles.push_back(comp["cutoff_angles"][i].as_double("cutoff_angles"));
+
+  c.energy = sub["energy"].as_double("energy");
+  c.wl = energy_to_wavelength(c.energy);
+  std::vector<DbEntry> table;
+  if (sub.has("deltabeta_table") && !sub["deltabeta_table"].is_null()) {
+    const YNode &t = sub["deltabeta_table"];
+    for (size_t i = 0; i < t.size(); ++i) {
+      const YNode &e = t[i];
+      DbEntry d;
+      d.name = e[0][0].as_string("deltabeta_table material");
+      d.density = e[0][1].as_double("deltabeta_table density");
+      d.db = {e[1][0].as_double("delta"), e[1][1].as_double("beta")};
+      table.push_back(d);
+    }
+  }
+  const YNode &src = sub["source"];
+  const std::string st = src["type"].as_string("source.type");
+  if (st == "point") {
+    c.source.kind = SourceCfg::Point;
+    c.source.x = get_f(src, "x");
+    c.source.z = get_f(src, "z");
+  } else if (st == "vector") {
+    c.source.kind = SourceCfg::Vector;
+    c.source.input_path = src["input_path"].as_string("input_path");
+    c.source.z = get_f(src, "z");
+  } else {
+    throw Error(2, "Unknown source type: " + st + ".");
+  }
+
+  const YNode &els = cfg["elements"];
+  for (size_t i = 0; i < els.size(); ++i) {
+    const YNode &e = els[i];
+    const std::string type = e["type"].as_string("type");
+    if (type != "grating")
+      throw Error(6, "optical element type '" + type + "' is not supported by this build (grating only)");
+    GratingCfg g;
+    g.pitch = get_f(e, "pitch");
+    g.dc0 = e["dc"][0].as_double("dc.0");
+    g.dc1 = e["dc"][1].as_double("dc.1");
+    g.z_start = get_f(e, "z_start");
+    g.thickness = get_f(e, "thickness");
+    g.nr_steps = (int)e["nr_steps"].as_int("nr_steps");
+    g.x_positions.clear();
+    const YNode &xp = e["x_positions"];
+    for (size_t k = 0; k < xp.size(); ++k) g.x_positions.push_back(xp[k].as_double("x_positions"));
+    g.substrate_thickness = get_f(e, "substrate_thickness");
+    auto mat = [&](const char *k) -> std::complex<double> {
+      if (!e.has(k)) return {0, 0};
+      return lookup(table, e[k], k);
+    };
+    g.db_a = mat("mat_a");
+    g.db_b = mat("mat_b");
+    g.db_s = mat("mat_substrate");
+    c.elements.push_back(g);
+  }
+  return c;
+}
+
+void check_config(const SimConfig &c) {
+  if (c.N < 64 || (c.N & (c.N - 1)) != 0) throw Error(6, "N must be a power of two >= 64");
+  if (!(c.dx > 0)) throw Error(2, "dx must be positive");
+  if (c.chunk_size <= 0) throw Error(2, "chunk_size must be positive");
+  if (!(c.px > 0) || !(c.detector_size > 0)) throw Error(2, "Detector size and pixel size must be positive");
+  if (detector_pixels(c.detector_size, c.px) > c.N) throw Error(2, "detector has more pixels than samples");
+  if (c.cutoff_angles.size() != c.elements.size() + 1)
+    throw Error(2, "cutoff_angles must have len(elements) + 1 entries");
+  const double tol = 1e-8;
+  double cur = c.source.z;
+  size_t nsteps = 0;
+  for (const auto &g : c.elements) {
+    if (!(g.thickness > 0)) throw Error(2, "grating thickness must be positive");
+    if (g.substrate_thickness < 0) throw Error(2, "substrate thickness must be >= 0");
+    if (g.nr_steps < 1) throw Error(2, "nr_steps must be >= 1");
+    if (g.x_positions.empty()) throw Error(2, "x_positions must not be empty");
+    if (g.x_positions.size() > 1) {
+      if (nsteps && nsteps != g.x_positions.size())
+        throw Error(2, "If multiple elements have phase stepping enabled, then all elements must have the same number of steps.");
+      nsteps = g.x_positions.size();
+    }
+    if (!(cur <= g.z_start + tol)) throw Error(2, "Overlapping optical elements are not supported");
+    cur = g.z_start + g.total_thickness();
+  }
+  if (!(cur <= c.z_detector + tol)) throw Error(2, "Detector must be after the last optical element");
+}
+
+namespace {
+
+XOp mask_op(const SimConfig &c, const GratingCfg &g, int slice, int step, const double fa[2], const double fb[2]) {
+  XOp x{};
+  x.kind = XOP_MASK;
+  x.fa[0] = fa[0];
+  x.fa[1] = fa[1];
+  x.fb[0] = fb[0];
+  x.fb[1] = fb[1];
+  x.post[0] = 1.0;
+  x.post[1] = 0.0;
+  // big-wave/optical_element.py:156, 351: dc interpolated in FP64
+  const double zf = (double)slice / (double)g.nr_steps;
+  const double dc = (1.0 - zf) * g.dc0 + zf * g.dc1;
+  x.pitch = g.pitch;
+  x.dc = dc;
+  x.half_dc = dc / 2.0;
+  x.xpos = g.x_positions[step];
+  x.dx = c.dx;
+  x.chunk = c.chunk_size;
+  x.single_chunk = c.N <= c.chunk_size;
+  return x;
+}
+
+}  // namespace
+
+void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops) {
+  const double zs = g.thickness / (double)g.nr_steps;
+  double fa[2], fb[2];
+  material_factor(g.db_a.real(), g.db_a.imag(), zs, c.wl, fa);
+  material_factor(g.db_b.real(), g.db_b.imag(), zs, c.wl, fb);
+  for (int i = 0; i < g.nr_steps; ++i) {
+    Op m{OpKind::Mask};
+    m.x = mask_op(c, g, i, step, fa, fb);
+    ops.push_back(m);
+    Op p{OpKind::Prop};
+    p.dz = zs;
+    p.fc = fc;
+    ops.push_back(p);
+  }
+  if (g.substrate_thickness > 0) {
+    double fs[2];
+    material_factor(g.db_s.real(), g.db_s.imag(), g.substrate_thickness, c.wl, fs);
+    Op m{OpKind::Mask};
+    m.x.kind = XOP_SCALAR;
+    m.x.fa[0] = fs[0];
+    m.x.fa[1] = fs[1];
+    m.x.post[0] = 1.0;
+    m.x.post[1] = 0.0;
+    ops.push_back(m);
+    Op p{OpKind::Prop};
+    p.dz = g.substrate_thickness;
+    p.fc = fc;
+    ops.push_back(p);
+  }
+}
+
+namespace {
+
+void grid_density_check(double dz, double xs, int64_t N, double dx, double wl) {
+  // big-wave/propagation.py:31-45
+  auto dist = [&](double idx) {
+    const double a = ((double)N / 2.0 - idx) * dx + std::fabs(xs);
+    return std::sqrt(a * a + dz * dz);
+  };
+  if (!(std::fabs(dist(0) - dist(1)) <= wl / 2.0))
+    throw Error(2, "Nyquist not satisfied, dx too large for the point source");
+}
+
+}  // namespace
+
+int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
+  check_config(c);
+  m.wl = c.wl;
+  m.out_base = out_base;
+  std::vector<Op> &ops = m.ops;
+  ops.clear();
+  std::vector<double> fcs;
+  for (double a : c.cutoff_angles) fcs.push_back(std::sin(a) / c.wl);
+  const double tol = 1e-8;
+
+  double fc = fcs[0];
+  double cur = c.elements.empty() ? c.z_detector : c.elements[0].z_start;
+  // source (big-wave/source.py:46-102)
+  if (c.source.kind == SourceCfg::Point) {
+    const double dz = cur - c.source.z;
+    grid_density_check(dz, c.source.x, c.N, c.dx, c.wl);
+    Op s{OpKind::Source};
+    s.x.kind = XOP_POINT;
+    s.x.post[0] = 1.0;
+    s.x.xs = c.source.x;
+    s.x.z = dz;
+    s.x.dx = c.dx;
+    s.x.kphase = -2.0 * M_PI / c.wl;
+    ops.push_back(s);
+    Op p{OpKind::Prop};
+    p.dz = 0.0;
+    p.fc = fc;
+    ops.push_back(p);
+  } else {
+    if (c.source.kind == SourceCfg::Plane) {
+      Op s{OpKind::Source};
+      s.x.kind = XOP_ONES;
+      s.x.post[0] = 1.0;
+      ops.push_back(s);
+    } else {
+      ops.push_back(Op{OpKind::Upload});
+    }
+    const double dz = cur - c.source.z;
+    if (dz != 0) {
+      Op p{OpKind::Prop};
+      p.dz = dz;
+      p.fc = fc;
+      ops.push_back(p);
+    }
+  }
+
+  int snap_idx = -1;
+  double snap_z = 0;
+  size_t nsteps = 1;
+  for (size_t i = 0; i < c.elements.size(); ++i) {
+    const GratingCfg &g = c.elements[i];
+    if (cur < g.z_start) {
+      Op p{OpKind::Prop};
+      p.dz = g.z_start - cur;
+      p.fc = fc;
+      ops.push_back(p);
+    }
+    cur = g.z_start;
+    if (g.x_positions.size() > 1 && snap_idx < 0) {
+      snap_idx = (int)i;
+      snap_z = cur;
+      nsteps = g.x_positions.size();
+      ops.push_back(Op{OpKind::Snapshot});
+    }
+    fc = fcs[i + 1];
+    grating_ops(c, g, 0, fc, ops);
+    cur = g.z_start + g.total_thickness();
+  }
+  auto final_hop = [&]() {
+    if (cur < c.z_detector - tol) {
+      Op p{OpKind::Prop};
+      p.dz = c.z_detector - cur;
+      p.fc = fc;
+      ops.push_back(p);
+    }
+  };
+  final_hop();
+  {
+    Op d{OpKind::Detect};
+    d.row = 0;
+    ops.push_back(d);
+    if (save_u) {
+      Op s{OpKind::SaveU};
+      s.row = 0;
+      ops.push_back(s);
+    }
+  }
+  if (snap_idx >= 0) {
+    for (size_t step = 1; step < nsteps; ++step) {
+      ops.push_back(Op{OpKind::Restore});
+      cur = snap_z;
+      fc = fcs[snap_idx];
+      for (size_t i = snap_idx; i < c.elements.size(); ++i) {
+        const GratingCfg &g = c.elements[i];
+        if (cur < g.z_start) {
+          Op p{OpKind::Prop};
+          p.dz = g.z_start - cur;
+          p.fc = fc;
+          ops.push_back(p);
+        }
+        cur = g.z_start;
+        fc = fcs[i + 1];
+        grating_ops(c, g, g.x_positions.size() > 1 ? (int)step : 0, fc, ops);
+        cur = g.z_start + g.total_thickness();
+      }
+      final_hop();
+      Op d{OpKind::Detect};
+      d.row = (int)step;
+      ops.push_back(d);
+      if (save_u) {
+        Op s{OpKind::SaveU};
+        s.row = (int)step;
+        ops.push_back(s);
+      }
+    }
+  }
+  return (int)nsteps;
+}
+
+}  // namespace rs

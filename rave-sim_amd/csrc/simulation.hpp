@@ -1,0 +1,59 @@
+// Sim-dir configuration and per-source driver, mirroring fast-wave's
+// include/simulation.hpp + src/config_parsing.cpp and big-wave's
+// wavesim.run_simulation (big-wave/wavesim.py:84-298).  Where the two
+// references differ, big-wave's semantics are followed.
+#pragma once
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace rs {
+
+struct GratingCfg {
+  double pitch = 0, dc0 = 0.5, dc1 = 0.5, z_start = 0, thickness = 0;
+  int nr_steps = 1;
+  std::vector<double> x_positions{0.0};
+  double substrate_thickness = 0;
+  std::complex<double> db_a{0, 0}, db_b{0, 0}, db_s{0, 0};
+  double total_thickness() const { return thickness + substrate_thickness; }
+};
+
+struct SourceCfg {
+  enum Kind { Point, Vector, Plane } kind = Point;
+  double x = 0, z = 0;
+  std::string input_path;
+};
+
+struct SimConfig {
+  int64_t N = 0;
+  double dx = 0, z_detector = 0, detector_size = 0, px = 0, py = 1;
+  int64_t chunk_size = 0;
+  bool f64 = true;               // big-wave default dtype c16 (big-wave/config.py:176-177)
+  bool save_final_u_vectors = false;
+  std::vector<GratingCfg> elements;
+  std::vector<double> cutoff_angles;
+  double energy = 0, wl = 0;
+  SourceCfg source;
+};
+
+std::string subdir_name(int source_idx);  // "%08d"
+
+// Parse sim_dir/config.yaml, computed.yaml and %08d/subconfig.yaml
+// (fast-wave/src/config_parsing.cpp:281-307, big-wave/multisim.py:70-112).
+SimConfig parse_simdir(const std::string &sim_dir, int source_idx);
+
+// Build the member program of one source, incl. phase steps.  Returns the
+// number of detector rows (phase steps).
+int build_member(const SimConfig &c, Member &m, int out_base, bool save_u);
+
+// Slice loop + substrate of one grating as member ops, using
+// x_positions[step] (big-wave/optical_element.py:134-206).
+void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops);
+
+// Check element ordering / phase-step consistency like big-wave's
+// check_simulation_inputs (big-wave/multisim.py:195-242); throws Error.
+void check_config(const SimConfig &c);
+
+}  // namespace rs

@@ -1,0 +1,225 @@
+"""Context, field operators and the batched runner over the C-ABI."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+_H = 6.62607004 * 10 ** (-34)
+_C0 = 299792458
+_EV = 1.602176634 * 10 ** (-19)
+
+
+def energy_to_wavelength(e: float) -> float:
+    """lambda = h c / (E e) with big-wave's constants (big-wave/propagation.py:26-28, 69-71)."""
+    return _H * _C0 / (e * _EV)
+
+
+class Context:
+    """One HIP device and its stream (rs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib.rs_ctx_create(device, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if self._h is None:
+            raise _lib.RsError(_lib.RS_ERR_INVALID, "context is closed")
+        return self._h
+
+    def synchronize(self) -> None:
+        check(lib.rs_ctx_synchronize(self.handle))
+
+    def set_timing(self, enable: bool) -> None:
+        """Per-launch HIP-event timing of the fused passes (resets counters)."""
+        check(lib.rs_set_timing(self.handle, 1 if enable else 0))
+
+    def pass_stats(self) -> dict:
+        """{class: (launches, total_ms, algorithmic bytes)} for row / top / mid / other."""
+        arr = (_lib.rs_pass_stat * 4)()
+        check(lib.rs_get_pass_stats(self.handle, arr, 4))
+        names = ("row", "top", "mid", "other")
+        return {n: (a.launches, a.total_ms, a.bytes) for n, a in zip(names, arr)}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib.rs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+@dataclass
+class Grating:
+    """Grating element, fields as in big-wave/optical_element.py:116-130 with
+    deltabeta values resolved (0 = vacuum)."""
+
+    pitch: float
+    dc: tuple
+    z_start: float
+    thickness: float
+    nr_steps: int
+    x_positions: Sequence[float] = (0.0,)
+    substrate_thickness: float = 0.0
+    db_a: complex = 0j
+    db_b: complex = 0j
+    db_substrate: complex = 0j
+
+    def to_c(self):
+        xs = (C.c_double * len(self.x_positions))(*[float(v) for v in self.x_positions])
+        g = _lib.rs_grating(
+            pitch=self.pitch, dc0=self.dc[0], dc1=self.dc[1], thickness=self.thickness,
+            nr_steps=self.nr_steps, nr_x_positions=len(self.x_positions),
+            x_positions=C.cast(xs, C.POINTER(C.c_double)),
+            substrate_thickness=self.substrate_thickness,
+            db_a_re=complex(self.db_a).real, db_a_im=complex(self.db_a).imag,
+            db_b_re=complex(self.db_b).real, db_b_im=complex(self.db_b).imag,
+            db_s_re=complex(self.db_substrate).real, db_s_im=complex(self.db_substrate).imag)
+        return g, xs
+
+    def get_thickness(self) -> float:
+        return self.thickness + self.substrate_thickness
+
+
+class Field:
+    """A device-resident complex field (rs_field) with big-wave's operators."""
+
+    def __init__(self, ctx: Context, N: int, dx: float, dtype=np.complex64,
+                 chunk_size: Optional[int] = None, plain: bool = False):
+        self.ctx = ctx
+        self.N, self.dx = int(N), float(dx)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.complex64, np.complex128):
+            raise ValueError("dtype must be complex64 or complex128")
+        self._grid = _lib.rs_grid(N=self.N, dx=self.dx, chunk_size=int(chunk_size or N))
+        h = C.c_void_p()
+        code = _lib.RS_C8 if self.dtype == np.complex64 else _lib.RS_C16
+        mode = _lib.RS_MODE_PLAIN if plain else _lib.RS_MODE_FUSED
+        check(lib.rs_field_create(ctx.handle, C.byref(self._grid), code, mode, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib.rs_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, u: np.ndarray) -> None:
+        a = np.ascontiguousarray(u, dtype=self.dtype)
+        if a.shape != (self.N,):
+            raise ValueError(f"expected shape ({self.N},)")
+        check(lib.rs_field_upload(self._h, a.ctypes.data_as(C.c_void_p)))
+
+    def download(self) -> np.ndarray:
+        a = np.empty(self.N, dtype=self.dtype)
+        check(lib.rs_field_download(self._h, a.ctypes.data_as(C.c_void_p)))
+        return a
+
+    def point_source(self, wl: float, x_source: float, z: float) -> None:
+        check(lib.rs_point_source(self._h, wl, x_source, z))
+
+    def propagate(self, wl: float, dz: float, cutoff_freq: float = math.inf) -> None:
+        check(lib.rs_propagate(self._h, wl, dz, cutoff_freq))
+
+    def apply_grating(self, wl: float, g: Grating, phase_step: int = 0,
+                      cutoff_freq: float = math.inf) -> None:
+        cg, keep = g.to_c()
+        check(lib.rs_apply_grating(self._h, wl, C.byref(cg), phase_step, cutoff_freq))
+        del keep
+
+    def scale(self, factor: complex) -> None:
+        check(lib.rs_scale(self._h, complex(factor).real, complex(factor).imag))
+
+    def square_and_downsample(self, size: float, px: float, py: float, z: float) -> np.ndarray:
+        det = _lib.rs_detector(size=size, pixel_x=px, pixel_y=py)
+        n = lib.rs_detector_pixels(C.byref(det))
+        out = np.zeros(max(n, 0), dtype=np.float64)
+        check(lib.rs_square_and_downsample(self._h, C.byref(det), z,
+                                           out.ctypes.data_as(C.POINTER(C.c_double)), len(out)))
+        return out
+
+
+def detector_bounds(N: int, dx: float, size: float, px: float) -> np.ndarray:
+    """Per-pixel sample boundaries (host computation, no device)."""
+    det = _lib.rs_detector(size=size, pixel_x=px, pixel_y=1.0)
+    n = lib.rs_detector_pixels(C.byref(det))
+    b = np.zeros(n + 1, dtype=np.int64)
+    check(lib.rs_detector_bounds(N, dx, C.byref(det), b.ctypes.data_as(C.POINTER(C.c_int64)), len(b)))
+    return b
+
+
+@dataclass
+class BatchMember:
+    """One (energy, source) simulation of a batch."""
+
+    wl: float
+    gratings: Sequence[Grating]
+    cutoff_angles: Sequence[float]
+    weight: float = 1.0
+    out_index: int = 0
+    plane_wave: bool = False
+    source_x: float = 0.0
+    source_z: float = 0.0
+
+
+def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *,
+              det_size: float, det_px: float, det_py: float, z_detector: float,
+              n_outputs: int = 1, dtype=np.complex64, chunk_size: Optional[int] = None,
+              plain: bool = False, max_resident: int = 0, use_graph: bool = False) -> np.ndarray:
+    """Run every member in lockstep batches; returns (n_outputs, n_px) float64
+    with sum over members of weight * detector row."""
+    if not members:
+        raise ValueError("no members")
+    ng = len(members[0].gratings)
+    keep = []
+    cm = (_lib.rs_batch_member * len(members))()
+    for i, m in enumerate(members):
+        if len(m.gratings) != ng or len(m.cutoff_angles) != ng + 1:
+            raise ValueError("all members need the same number of gratings and ng+1 cutoff angles")
+        garr = (_lib.rs_grating * max(ng, 1))()
+        for k, g in enumerate(m.gratings):
+            cg, xs = g.to_c()
+            garr[k] = cg
+            keep.append(xs)
+        ang = (C.c_double * (ng + 1))(*[float(a) for a in m.cutoff_angles])
+        keep += [garr, ang]
+        cm[i] = _lib.rs_batch_member(wl=m.wl, weight=m.weight, source_kind=1 if m.plane_wave else 0,
+                                     out_index=m.out_index, source_x=m.source_x, source_z=m.source_z,
+                                     gratings=C.cast(garr, C.POINTER(_lib.rs_grating)),
+                                     cutoff_angles=C.cast(ang, C.POINTER(C.c_double)))
+    desc = _lib.rs_batch_desc(
+        grid=_lib.rs_grid(N=N, dx=dx, chunk_size=int(chunk_size or N)),
+        det=_lib.rs_detector(size=det_size, pixel_x=det_px, pixel_y=det_py),
+        z_detector=z_detector,
+        dtype=_lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16,
+        mode=_lib.RS_MODE_PLAIN if plain else _lib.RS_MODE_FUSED,
+        nr_gratings=ng, nr_members=len(members), members=cm, nr_outputs=n_outputs,
+        max_resident=max_resident, use_graph=1 if use_graph else 0, reserved=0)
+    det = _lib.rs_detector(size=det_size, pixel_x=det_px, pixel_y=1.0)
+    npx = lib.rs_detector_pixels(C.byref(det))
+    out = np.zeros((n_outputs, npx), dtype=np.float64)
+    check(lib.rs_batch_run(ctx.handle, C.byref(desc), out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+    return out

@@ -1,0 +1,97 @@
+"""Seeded synthetic inputs for the benchmark configurations, as sim dirs.
+
+Each builder returns (config dict, list of sub-configs, cutoff angles) in
+big-wave's sim-dir schema.  Full-size and reduced-size variants share the
+physics (energies, gratings, distances); reduced variants shrink N, slices and
+the detector so that the complex128 oracle finishes in seconds.
+
+Builder decisions (BASELINE leaves them open): chunk_size = N unless given,
+dx = 2 nm, plane waves are a vector source of ones at the first grating,
+no frequency cutoff (all cutoff angles pi/2) for plane-wave chains, the
+detector follows the last element directly when the config says so.
+"""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+
+# delta/beta of the configurations
+DB_SI_25KEV = (7.7e-7, 3.5e-10)
+DB_AU_46KEV = (1.1e-6, 4.0e-8)
+DB_SI_46KEV = (2.6e-7, 1.0e-10)
+
+
+def grating(pitch, z_start, thickness, nr_steps, mat_a, x_positions=(0.0,), dc=(0.5, 0.5),
+            substrate=0.0, mat_b=None, mat_s=None):
+    return {"type": "grating", "pitch": pitch, "dc": list(dc), "z_start": z_start,
+            "thickness": thickness, "nr_steps": nr_steps, "x_positions": [float(x) for x in x_positions],
+            "substrate_thickness": substrate, "mat_a": mat_a, "mat_b": mat_b, "mat_substrate": mat_s}
+
+
+def sim_params(N, dx, z_det, det_size, px, py=1.0, chunk=None):
+    return {"N": int(N), "dx": dx, "z_detector": z_det, "detector_size": det_size,
+            "detector_pixel_size_x": px, "detector_pixel_size_y": py, "chunk_size": int(chunk or N)}
+
+
+def write_ones(path: Path, N: int) -> str:
+    np.save(path, np.ones(N, dtype=np.complex128))
+    return str(path)
+
+
+def cfg1(tmp: Path, N=2 ** 20, slices=32, det_size=1e-3, dtype="c16", chunk=None):
+    """Plain monochromatic chain: plane wave at 25 keV, one Si phase grating
+    (p 4.2 um, h 30 um, 32 slices), 0.1 m hop, 1 um pixels."""
+    dx = 2e-9
+    h = 30e-6
+    cfg = {"sim_params": sim_params(N, dx, h + 0.1, det_size, 1e-6, 1.0, chunk),
+           "use_disk_vector": False, "save_final_u_vectors": False, "dtype": dtype,
+           "elements": [grating(4.2e-6, 0.0, h, slices, ["Si", 2.34])]}
+    src = write_ones(tmp / "ones.npy", N)
+    subs = [{"energy": 25000.0, "source": {"type": "vector", "input_path": src, "z": 0.0},
+             "deltabeta_table": [[["Si", 2.34], list(DB_SI_25KEV)]]}]
+    return cfg, subs, [math.pi / 2] * 2
+
+
+def cfg2(tmp: Path, N=2 ** 26, slices=64, det_size=None, dtype="c8", chunk=None, g2_steps=1):
+    """Talbot-Lau 3-grating chain at 46 keV: G0 Au absorption, G1 Si phase,
+    G2 = G0, 60 um high, 0.2 m start to start; plane wave at G0, detector at
+    the end of G2.  g2_steps > 1 phase-steps G2 at k p / g2_steps."""
+    dx = 2e-9
+    h, p = 60e-6, 4.2e-6
+    if det_size is None:
+        det_size = min(1e-3, N * dx / 2)
+    xs = [k * p / g2_steps for k in range(g2_steps)]
+    els = [grating(p, 0.0, h, slices, ["Au", 19.32]),
+           grating(p, 0.2, h, slices, ["Si", 2.34]),
+           grating(p, 0.4, h, slices, ["Au", 19.32], x_positions=xs)]
+    cfg = {"sim_params": sim_params(N, dx, 0.4 + h, det_size, 1e-6, 1.0, chunk),
+           "use_disk_vector": False, "save_final_u_vectors": False, "dtype": dtype, "elements": els}
+    src = write_ones(tmp / "ones.npy", N)
+    subs = [{"energy": 46000.0, "source": {"type": "vector", "input_path": src, "z": 0.0},
+             "deltabeta_table": [[["Au", 19.32], list(DB_AU_46KEV)], [["Si", 2.34], list(DB_SI_46KEV)]]}]
+    return cfg, subs, [math.pi / 2] * 4
+
+
+def spectrum_weights(energies_kev):
+    """Synthetic 70 kVp tungsten spectrum: Kramers (70-E)/E times 3 mm Al with
+    mu_Al = 0.5/cm (E/50)^-3, normalised to sum 1."""
+    e = np.asarray(energies_kev, dtype=np.float64)
+    mu = 0.5 * (e / 50.0) ** -3
+    w = (70.0 - e) / e * np.exp(-mu * 0.3)
+    w = np.clip(w, 0, None)
+    return w / w.sum()
+
+
+def cfg3_members(n_energies=64, N=2 ** 24, slices=32):
+    """Polychromatic cfg-1 chain: energies 20..70 keV uniformly, delta ~ E^-2,
+    beta ~ E^-4 scaled from the 25 keV values."""
+    e = np.linspace(20.0, 70.0, n_energies)
+    w = spectrum_weights(e)
+    out = []
+    for ek, wk in zip(e, w):
+        s = 25.0 / ek
+        db = complex(DB_SI_25KEV[0] * s ** 2, DB_SI_25KEV[1] * s ** 4)
+        out.append((ek * 1000.0, float(wk), db))
+    return out

@@ -1,0 +1,171 @@
+"""GPU parity of the HIP path against the complex128 oracle (big-wave
+restatement) on the same seeded inputs.
+
+Tolerances: complex128 fields (c16) to ~1e-12 relative; complex64 fields
+with FP64 phase arguments (c8) to the 1e-4 relative-L2 intensity bound and
+1e-3 rad fringe phase of the precision form; integer / mask / pixel-range
+work bit exact."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import bigwave as bw
+from simcases import DB_SI_25KEV, cfg1, cfg2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import ravesim
+    c = ravesim.Context(0)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rand_field(N, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(N) + 1j * rng.standard_normal(N)
+
+
+# N covering one, two and three digit groups for each precision
+PROP_CASES = [(2 ** 10, np.complex128), (2 ** 12, np.complex64), (2 ** 16, np.complex128),
+              (2 ** 18, np.complex64), (2 ** 22, np.complex128), (2 ** 23, np.complex64)]
+
+
+@pytest.mark.parametrize("N,dtype", PROP_CASES)
+@pytest.mark.parametrize("plain", [False, True])
+def test_propagate_matches_oracle(ctx, N, dtype, plain):
+    import ravesim
+    dx, wl, dz = 2e-9, 2.7e-11, 1.7e-3
+    u = rand_field(N, N)
+    fc = 0.3 / dx  # removes part of the band
+    f = ravesim.Field(ctx, N, dx, dtype=dtype, plain=plain)
+    f.upload(u)
+    f.propagate(wl, dz, fc)
+    got = f.download()
+    ref = bw.Fields(u.astype(np.complex128))
+    bw.propagate(ref, dx, wl, dz, N, fc)
+    tol = 1e-11 if dtype == np.complex128 else 2e-6
+    assert rel_l2(got, ref.u) < tol
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_two_hops_equal_one(ctx, dtype):
+    import ravesim
+    N, dx, wl = 2 ** 20, 2e-9, 2.7e-11
+    u = rand_field(N, 5)
+    a = ravesim.Field(ctx, N, dx, dtype=dtype)
+    a.upload(u)
+    a.propagate(wl, 0.03)
+    a.propagate(wl, 0.07)
+    b = ravesim.Field(ctx, N, dx, dtype=dtype)
+    b.upload(u)
+    b.propagate(wl, 0.10)
+    tol = 1e-10 if dtype == np.complex128 else 1e-5
+    assert rel_l2(a.download(), b.download()) < tol
+
+
+@pytest.mark.parametrize("N,chunk,xpos", [(2 ** 14, None, 0.0), (2 ** 16, 1000, 3.3e-7),
+                                          (2 ** 20, 10469376 // 16, -1.05e-6)])
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_grating_mask_bit_exact(ctx, N, chunk, xpos, dtype):
+    """pitch/dx = 2100 puts samples exactly on mask edges; multi-chunk x."""
+    import ravesim
+    dx, wl = 2e-9, 2.7e-11
+    t = 1e-12  # negligible propagation; material A halves the amplitude
+    beta = math.log(2) * wl / (2 * math.pi * t)
+    g = ravesim.Grating(pitch=4.2e-6, dc=(0.37, 0.62), z_start=0.0, thickness=t * 3, nr_steps=3,
+                        x_positions=[xpos, xpos + 1e-7], db_a=complex(0, beta))
+    chunk = chunk or N
+    f = ravesim.Field(ctx, N, dx, dtype=dtype, chunk_size=chunk)
+    f.upload(np.ones(N))
+    og = bw.Grating(pitch=4.2e-6, dc=(0.37, 0.62), z_start=0.0, thickness=t * 3, nr_steps=3,
+                    x_positions=np.array([xpos, xpos + 1e-7]), substrate_thickness=0.0)
+    # one slice at a time: compare the mask of each slice separately
+    g1 = ravesim.Grating(pitch=4.2e-6, dc=(0.37, 0.37), z_start=0.0, thickness=t, nr_steps=1,
+                         x_positions=[xpos, xpos + 1e-7], db_a=complex(0, beta))
+    f.apply_grating(wl, g1, phase_step=1)
+    got = np.abs(f.download()) < 0.75
+    exp = np.concatenate([bw.grating_mask(dx, i - N // 2, min(chunk, N - i), og, 0.0, 1)
+                          for i in range(0, N, chunk)])
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_point_source_matches_oracle(ctx, dtype):
+    import ravesim
+    N, dx, wl = 2 ** 16, 3.1e-10, 2.67e-11
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.point_source(wl, 4.4e-7, 0.1)
+    ref = bw.propagate_analytically(N, dx, wl, N, 0.1, 4.4e-7)
+    tol = 1e-13 if dtype == np.complex128 else 1e-6
+    assert rel_l2(f.download(), ref) < tol
+
+
+@pytest.mark.parametrize("N,size,px", [(8, 3.0, 1.0), (2 ** 16, 1e-4, 1e-6), (2 ** 16, 5e-5, 3e-7),
+                                       (2 ** 12, 4e-6, 1e-7)])
+def test_detector_matches_histogram(ctx, N, size, px):
+    import ravesim
+    dx = 0.5 if N == 8 else 2e-9
+    u = np.arange(8, dtype=np.complex128) if N == 8 else rand_field(N, 11)
+    if N < 64:
+        pytest.skip("fields need N >= 64")
+    f = ravesim.Field(ctx, N, dx, dtype=np.complex128)
+    f.upload(u)
+    got = f.square_and_downsample(size, px, 1.0, 1.0)
+    p = bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=size, detector_pixel_size_x=px,
+                     detector_pixel_size_y=1.0, wl=1.0, chunk_size=N)
+    exp = bw.square_and_downsample(u, p, 1.0)
+    assert got.shape == exp.shape
+    assert np.allclose(got, exp, rtol=1e-12, atol=0)
+
+
+def _simdir(tmp_path, case):
+    from ravesim.simdir import write_simdir
+    cfg, subs, ang = case
+    return write_simdir(tmp_path / "sim", cfg, subs, ang)
+
+
+@pytest.mark.parametrize("dtype", ["c16", "c8"])
+def test_cfg1_reduced_end_to_end(ctx, tmp_path, dtype):
+    from ravesim.multisim import run_single_simulation
+    sd = _simdir(tmp_path, cfg1(tmp_path, N=2 ** 16, slices=8, det_size=1e-4, dtype=dtype))
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    assert got.shape == ref.shape == (1, 100)
+    saved = np.load(sd / "00000000" / "detected.npy")
+    assert saved.dtype == np.float64 and np.array_equal(saved, got)
+    tol = 1e-11 if dtype == "c16" else 1e-4
+    assert rel_l2(got, ref) < tol
+
+
+def test_cfg2_reduced_phase_stepping(ctx, tmp_path):
+    from ravesim.multisim import run_single_simulation
+    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 18, slices=6, det_size=1e-4, dtype="c8", g2_steps=8))
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    assert got.shape == ref.shape == (8, 100)
+    assert rel_l2(got, ref) < 1e-4
+    k = np.arange(8)
+    def phase(I):
+        a_s = (I * np.sin(2 * np.pi * k / 8)[:, None]).sum(0)
+        a_c = (I * np.cos(2 * np.pi * k / 8)[:, None]).sum(0)
+        return -np.arctan2(a_s, a_c)
+    dphi = np.angle(np.exp(1j * (phase(got) - phase(ref))))
+    assert np.abs(dphi).max() < 1e-3
+
+
+def test_plain_and_fused_agree(ctx, tmp_path):
+    from ravesim.multisim import run_single_simulation
+    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 17, slices=3, det_size=1e-4, dtype="c16", g2_steps=2))
+    a = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    b = run_single_simulation(sd, 0, tmp_path, ctx=ctx, plain=True)
+    assert rel_l2(a, b) < 1e-11

@@ -1,0 +1,30 @@
+"""big-wave's end-to-end golden vector (big-wave/test.py:853-960) through the
+product path: sim dir written by ravesim.simdir.setup_simulation (delta/beta
+from the oracle's Chantler restatement), run by rs_run_simdir on the GPU."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import xray
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,atol", [("c16", 1e-14), ("c8", None)])
+def test_golden_vector_on_gpu(golden_dir, tmp_path, dtype, atol):
+    import ravesim
+    from ravesim.multisim import run_single_simulation
+    from ravesim.simdir import setup_simulation
+    d = json.loads((golden_dir / "bigwave_snapshot.json").read_text())
+    cfg = dict(d["known_config"])
+    cfg["dtype"] = dtype
+    sd = setup_simulation(cfg, tmp_path / "sim", lambda m, rho, e: xray.xray_delta_beta(m, rho, e))
+    ctx = ravesim.Context(0)
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)[0]
+    exp = np.array(d["known_data"])
+    if atol is not None:
+        assert np.allclose(got, exp, rtol=0, atol=atol)
+    else:
+        assert np.linalg.norm(got - exp) / np.linalg.norm(exp) < 1e-4
+    ctx.close()

@@ -1,0 +1,181 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every declared symbol, the C++ sim-dir parser / validator, the exact detector
+pixel ranges, and the sim-dir producer's setup rules against the oracle's."""
+import json
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import bigwave as bw
+from simcases import cfg1, cfg2
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_library_exports_every_declared_symbol():
+    from ravesim import _lib
+    header = (ROOT / "include" / "ravesim.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(rs_\w+)\(", header, re.M))
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(_lib.lib, name), name
+    assert _lib.lib.rs_version() >= 100
+
+
+def test_failure_without_device_is_an_error_code():
+    import ravesim
+    from ravesim import _lib
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(_lib.RsError) as e:
+        ravesim.Context(0)
+    assert e.value.code in (_lib.RS_ERR_DEVICE, _lib.RS_ERR_INVALID)
+
+
+def _write(tmp_path, case):
+    from ravesim.simdir import write_simdir
+    cfg, subs, ang = case
+    return write_simdir(tmp_path / "sim", cfg, subs, ang)
+
+
+def test_validate_cfg1_and_cfg2(tmp_path):
+    from ravesim.multisim import validate_simdir
+    sd = _write(tmp_path, cfg1(tmp_path, N=2 ** 16, slices=32, det_size=1e-4))
+    N, bits, steps, npx, nel, props = validate_simdir(sd, 0)
+    assert (N, bits, steps, npx, nel, props) == (2 ** 16, 16, 1, 100, 1, 33)
+    sd2 = _write(tmp_path / "b", cfg2(tmp_path, N=2 ** 18, slices=64, dtype="c8", g2_steps=8))
+    N, bits, steps, npx, nel, props = validate_simdir(sd2, 0)
+    assert (N, bits, steps, nel, props) == (2 ** 18, 8, 8, 3, 194)
+
+
+def test_parser_reads_ruamel_style_yaml(tmp_path):
+    """Block sequences indented under their key and compact nested lists, as
+    big-wave's ruamel dumper writes them (indent mapping=2, sequence=4, offset=2)."""
+    from ravesim.multisim import validate_simdir
+    sd = tmp_path / "sim"
+    (sd / "00000000").mkdir(parents=True)
+    np.save(tmp_path / "ones.npy", np.ones(1024, dtype=np.complex128))
+    (sd / "config.yaml").write_text(f"""# comment
+sim_params:
+  N: 1024
+  dx: 8e-10   # trailing comment
+  z_detector: 1.77
+  detector_size: 2e-7
+  detector_pixel_size_x: 1e-8
+  detector_pixel_size_y: 1e-8
+  chunk_size: 10469376
+use_disk_vector: False
+save_final_u_vectors: false
+dtype: c8
+elements:
+  - type: grating
+    pitch: 4.2e-06
+    dc:
+      - 0.5
+      - 0.5
+    z_start: 0.1
+    thickness: 0.00014
+    nr_steps: 2
+    x_positions: [0.0, 1.0e-6]
+    substrate_thickness: 0.00023
+    mat_a:
+      - Si
+      - 2.34
+    mat_b: ["Au", 19.32]
+    mat_substrate: null
+""")
+    (sd / "computed.yaml").write_text("cutoff_angles:\n  - 0.1\n  - 0.2\n")
+    (sd / "00000000" / "subconfig.yaml").write_text(f"""energy: 46000.0
+source:
+  type: vector
+  input_path: '{tmp_path / "ones.npy"}'
+  z: 0.1
+deltabeta_table:
+  - - - Si
+      - 2.34
+    - - 2.2888e-07
+      - 2.5847e-10
+  - - - Au
+      - 19.32
+    - - 1.4981e-06
+      - 3.6174e-08
+""")
+    N, bits, steps, npx, nel, props = validate_simdir(sd, 0)
+    assert (N, bits, steps, npx, nel) == (1024, 8, 2, int(2e-7 // 1e-8), 1)  # 19: Python floor division
+    assert props == 2 + 1 + 1  # two slices, substrate, final hop
+
+
+@pytest.mark.parametrize("mutation,needle", [
+    (lambda c, s: c["sim_params"].pop("dx"), "dx"),
+    (lambda c, s: c.__setitem__("dtype", "c32"), "Unknown dtype"),
+    (lambda c, s: c["elements"][0].__setitem__("z_start", -1.0), "Overlapping"),
+    (lambda c, s: s[0]["deltabeta_table"].clear(), "not found"),
+    (lambda c, s: c["elements"][0].__setitem__("type", "sample"), "not supported"),
+    (lambda c, s: c["sim_params"].__setitem__("N", 1000), "power of two"),
+])
+def test_config_errors_are_reported(tmp_path, mutation, needle):
+    from ravesim import _lib
+    from ravesim.multisim import validate_simdir
+    cfg, subs, ang = cfg1(tmp_path, N=2 ** 12, slices=2, det_size=4e-6)
+    mutation(cfg, subs)
+    sd = _write(tmp_path, (cfg, subs, ang))
+    with pytest.raises(_lib.RsError) as e:
+        validate_simdir(sd, 0)
+    assert needle in str(e.value)
+    assert e.value.code in (_lib.RS_ERR_CONFIG, _lib.RS_ERR_UNSUPPORTED)
+
+
+def test_missing_simdir_is_io_error(tmp_path):
+    from ravesim import _lib
+    from ravesim.multisim import validate_simdir
+    with pytest.raises(_lib.RsError) as e:
+        validate_simdir(tmp_path / "nothing", 0)
+    assert e.value.code == _lib.RS_ERR_IO
+
+
+@pytest.mark.parametrize("N,dx,size,px", [
+    (8, 0.5, 3.0, 1.0), (118, 0.2, 10.0, 0.3), (118, 0.2, 2.0, 0.6), (4096, 2e-9, 4e-6, 1e-7),
+    (4096, 3.1e-10, 2.5e-7, 1e-8), (10000, 1e-9, 5e-6, 7e-8), (2 ** 16, 2e-9, 1e-4, 1e-6)])
+def test_detector_bounds_reproduce_histogram(N, dx, size, px):
+    """Each sample's pixel from the host ranges equals np.histogram's bin."""
+    from ravesim.core import detector_bounds
+    b = detector_bounds(N, dx, size, px)
+    o = int(size // px)
+    assert len(b) == o + 1 and np.all(np.diff(b) >= 0)
+    pos = (np.arange(N) - N / 2) * dx
+    rng = (-(o / 2) * px, (o / 2 + 1) * px)
+    # integer weights make every pixel sum exact, so equality is bin equality
+    rs = np.random.default_rng(0)
+    w = rs.integers(1, 1000, N).astype(np.float64)
+    exp = np.histogram(pos, bins=o + 1, range=rng, weights=w)[0][:-1]
+    got = np.array([w[b[k]:b[k + 1]].sum() for k in range(o)])
+    assert np.array_equal(got, exp)
+    cnt = np.histogram(pos, bins=o + 1, range=rng)[0][:-1]
+    assert np.array_equal(np.diff(b), cnt)
+
+
+def test_setup_rules_match_oracle(tmp_path, golden_dir):
+    """Product sim-dir producer: same RNG draws and cutoff angles as the oracle."""
+    from ravesim.simdir import setup_simulation
+    import yaml
+    d = json.loads((golden_dir / "bigwave_snapshot.json").read_text())
+    cfg = d["known_config"]
+    sd = setup_simulation(cfg, tmp_path / "sim", lambda m, r, e: (1e-7, 1e-10))
+    comp = yaml.safe_load((sd / "computed.yaml").read_text())
+    sub = yaml.safe_load((sd / "00000000" / "subconfig.yaml").read_text())
+    ms = cfg["multisource"]
+    E, X = bw.draw_point_sources(ms["seed"], 1, ms["energy_range"], ms["x_range"])
+    assert sub["energy"] == float(E[0]) and sub["source"]["x"] == float(X[0])
+    els = bw.elements_from_config(cfg["elements"], [(("Si", 2.34), 0j), (("Au", 19.32), 0j)])
+    ang, _ = bw.compute_cutoff_angles(cfg["sim_params"]["detector_size"], cfg["sim_params"]["dx"],
+                                      ms["energy_range"], ms["z"], [e.z_start for e in els],
+                                      [e.get_thickness() for e in els], cfg["sim_params"]["z_detector"],
+                                      1e-6)
+    assert np.allclose(comp["cutoff_angles"], ang, rtol=1e-15, atol=0)
