@@ -331,8 +331,13 @@ struct PropAcc {
 void prop_accumulate(PropAcc &a, const Op &op, double wl, int64_t N, double dx) {
   const double L = (double)N * dx;
   a.alpha += wl * op.dz / (2.0 * L * L);
-  double b = -op.dz / wl;
-  b -= std::floor(b);
+  // global phase exactly as big-wave evaluates it in FP64,
+  // np.exp(-1j * 2 * np.pi / wl * dz) (big-wave/propagation.py:126), then
+  // reduced to revolutions in extended precision
+  const double th = (-(2.0 * M_PI) / wl) * op.dz;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  long double red = fmodl((long double)th, two_pi) / two_pi;
+  double b = (double)(red - floorl(red));
   a.beta += b;
   a.beta -= std::floor(a.beta);
   int64_t rs_, re_;
@@ -386,14 +391,16 @@ XOp compose(const XOp &a, const XOp &b) {
     r.post[1] = p1;
     return r;
   }
-  if (a.kind == XOP_SCALAR && b.kind == XOP_MASK) {
+  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) && b.kind == XOP_MASK) {
     XOp m = b;
-    const double s0 = a.fa[0] * a.post[0] - a.fa[1] * a.post[1];
-    const double s1 = a.fa[0] * a.post[1] + a.fa[1] * a.post[0];
+    // value of `a` at every sample: fa * post for a scalar, post for ones
+    const double s0 = a.kind == XOP_ONES ? a.post[0] : a.fa[0] * a.post[0] - a.fa[1] * a.post[1];
+    const double s1 = a.kind == XOP_ONES ? a.post[1] : a.fa[0] * a.post[1] + a.fa[1] * a.post[0];
     const double p0 = m.post[0] * s0 - m.post[1] * s1;
     const double p1 = m.post[0] * s1 + m.post[1] * s0;
     m.post[0] = p0;
     m.post[1] = p1;
+    if (a.kind == XOP_ONES || a.from_ones) m.from_ones = 1;
     return m;
   }
   throw Error(6, "two x-space masks without a propagation in between");

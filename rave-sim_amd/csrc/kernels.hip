@@ -361,11 +361,14 @@ __global__ void __launch_bounds__(C *(L / 16))
   }
 
   const bool xop = g.top && A.x.kind != XOP_NONE;
-  const bool gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES);
+  const bool gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones);
   T2 v[16];
   if (!gen) {
 #pragma unroll
     for (int m = 0; m < 16; ++m) v[m] = data[base + (int64_t)(q + m * TPF) * js];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
 
   if (!ROW && A.do_inv) {
@@ -421,10 +424,10 @@ __global__ void __launch_bounds__(256) xop_kernel(const FieldArgs *__restrict__ 
   using T2 = typename C2T<S>::t;
   const FieldArgs &A = args[blockIdx.y];
   T2 *data = (T2 *)A.data;
-  const bool gen = A.x.kind == XOP_POINT || A.x.kind == XOP_ONES;
+  const bool gen = A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones;
   for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
     T2 v;
-    if (gen) { v.x = 0; v.y = 0; } else v = data[n];
+    if (gen) { v.x = 1; v.y = 0; } else v = data[n];
     data[n] = apply_xop(A.x, n, N, v);
   }
 }
@@ -446,30 +449,36 @@ __global__ void __launch_bounds__(256) kop_kernel(const FieldArgs *__restrict__ 
 // detector
 // ---------------------------------------------------------------------------
 
+// One workgroup per pixel; the members of the launch are visited in order
+// inside the workgroup, so members that share an accumulator row add into it
+// without races and in a fixed order (results do not depend on scheduling).
 template <typename S>
-__global__ void __launch_bounds__(256) detect_kernel(const DetArgs *__restrict__ args,
+__global__ void __launch_bounds__(256) detect_kernel(const DetArgs *__restrict__ args, int nf,
                                                      const int64_t *__restrict__ bounds) {
   using T2 = typename C2T<S>::t;
   __shared__ double red[256];
-  const DetArgs &A = args[blockIdx.y];
-  const T2 *u = (const T2 *)A.data;
   const int k = blockIdx.x;
   const int64_t lo = bounds[k], hi = bounds[k + 1];
-  double s = 0.0;
-  for (int64_t j = lo + threadIdx.x; j < hi; j += 256) {
-    const T2 z = u[j];
-    s += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-#pragma unroll
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  for (int f = 0; f < nf; ++f) {
+    const DetArgs &A = args[f];
+    const T2 *u = (const T2 *)A.data;
+    double s = 0.0;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += 256) {
+      const T2 z = u[j];
+      s += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+    }
+    red[threadIdx.x] = s;
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double prev = A.accumulate ? A.acc[k] : 0.0;
-    A.acc[k] = prev + A.weight * red[0];
+#pragma unroll
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double prev = A.accumulate ? A.acc[k] : 0.0;
+      A.acc[k] = prev + A.weight * red[0];
+    }
+    __syncthreads();
   }
 }
 
@@ -565,7 +574,7 @@ hipError_t launch_kop(const FieldArgs *args, int nf, const Geom &g, hipStream_t 
 template <typename S>
 hipError_t launch_detect(const DetArgs *args, int nf, const int64_t *bounds, int64_t npx, hipStream_t st) {
   if (npx <= 0) return hipSuccess;
-  hipLaunchKernelGGL(detect_kernel<S>, dim3((unsigned)npx, nf), dim3(256), 0, st, args, bounds);
+  hipLaunchKernelGGL(detect_kernel<S>, dim3((unsigned)npx), dim3(256), 0, st, args, nf, bounds);
   return hipGetLastError();
 }
 

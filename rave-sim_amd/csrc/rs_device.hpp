@@ -28,6 +28,8 @@ enum XopKind : int32_t {
 struct XOp {
   int32_t kind;
   int32_t single_chunk;  // 1 when N <= chunk (grating x needs no chunk index)
+  int32_t from_ones;     // MASK / SCALAR applied to a plane wave of ones (input not read)
+  int32_t pad_;
   double fa[2], fb[2];
   double post[2];
   // grating (big-wave/optical_element.py:341-357)
