@@ -104,6 +104,7 @@ typedef struct rs_grid {
 typedef struct rs_grating {
   double pitch;
   double dc0, dc1;          /* duty cycle at the first and after the last slice */
+  double z_start;           /* start of the grating on the optical axis [m] */
   double thickness;         /* without substrate */
   int32_t nr_steps;
   int32_t nr_x_positions;

@@ -83,6 +83,7 @@ rs::GratingCfg grating_from(const rs_grating *g) {
   c.pitch = g->pitch;
   c.dc0 = g->dc0;
   c.dc1 = g->dc1;
+  c.z_start = g->z_start;
   c.thickness = g->thickness;
   c.nr_steps = g->nr_steps;
   c.x_positions.assign(g->x_positions, g->x_positions + g->nr_x_positions);

@@ -54,7 +54,7 @@ class rs_grid(C.Structure):
 
 class rs_grating(C.Structure):
     _fields_ = [("pitch", C.c_double), ("dc0", C.c_double), ("dc1", C.c_double),
-                ("thickness", C.c_double), ("nr_steps", C.c_int32),
+                ("z_start", C.c_double), ("thickness", C.c_double), ("nr_steps", C.c_int32),
                 ("nr_x_positions", C.c_int32), ("x_positions", C.POINTER(C.c_double)),
                 ("substrate_thickness", C.c_double),
                 ("db_a_re", C.c_double), ("db_a_im", C.c_double),

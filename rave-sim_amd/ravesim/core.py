@@ -86,7 +86,8 @@ class Grating:
     def to_c(self):
         xs = (C.c_double * len(self.x_positions))(*[float(v) for v in self.x_positions])
         g = _lib.rs_grating(
-            pitch=self.pitch, dc0=self.dc[0], dc1=self.dc[1], thickness=self.thickness,
+            pitch=self.pitch, dc0=self.dc[0], dc1=self.dc[1], z_start=self.z_start,
+            thickness=self.thickness,
             nr_steps=self.nr_steps, nr_x_positions=len(self.x_positions),
             x_positions=C.cast(xs, C.POINTER(C.c_double)),
             substrate_thickness=self.substrate_thickness,
