@@ -161,6 +161,38 @@ template <int R, bool INV, typename T2> __device__ __forceinline__ void dftR(T2 
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
+// w[r] = W^(r e) for r < R from the table W[j] = exp(-2 pi i j / len),
+// index mask len-1: reads e, 2e, 4e, 8e and forms the other powers as one
+// product of two read values (error <= ~2 ulp, no long recurrences).
+template <typename T2, int R>
+__device__ __forceinline__ void twiddle_powers(const T2 *__restrict__ tab, int64_t e, int64_t mask, T2 *w) {
+  w[0].x = 1;
+  w[0].y = 0;
+  if constexpr (R >= 2) w[1] = tab[e & mask];
+  if constexpr (R >= 4) {
+    w[2] = tab[(2 * e) & mask];
+    w[3] = cmul(w[2], w[1]);
+  }
+  if constexpr (R >= 8) {
+    w[4] = tab[(4 * e) & mask];
+    w[5] = cmul(w[4], w[1]);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+  }
+  if constexpr (R >= 16) {
+    w[8] = tab[(8 * e) & mask];
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+  }
+}
+
+// split-table twiddle W_M^e = hi[e >> sh] * lo[e & (2^sh - 1)], M = 2^lgM
+template <typename T2>
+__device__ __forceinline__ T2 split_twiddle(const T2 *__restrict__ hi, const T2 *__restrict__ lo, int sh,
+                                            int64_t e) {
+  return cmul(hi[e >> sh], lo[e & ((int64_t(1) << sh) - 1)]);
+}
+
 template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV>
 __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
                                                const T2 *__restrict__ twf) {
@@ -176,12 +208,13 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
       else x[i][r] = fib[lds_pad(b + r * (L / R))];
     }
     if constexpr (!FIRST) {
-      const int k = b % NS;
+      // W_{NS R}^{r k} for r < R from a few table reads (k, 2k, 4k, 8k);
+      // the remaining powers are products of at most two of them
+      const int e = (b % NS) * (L / (NS * R));
+      T2 w[R];
+      twiddle_powers<T2, R>(twf, e, L - 1, w);
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const T2 w = twf[(r * k) * (L / (NS * R))];
-        x[i][r] = INV ? cmulc(x[i][r], w) : cmul(x[i][r], w);
-      }
+      for (int r = 1; r < R; ++r) x[i][r] = INV ? cmulc(x[i][r], w[r]) : cmul(x[i][r], w[r]);
     }
     dftR<R, INV>(x[i]);
   }
@@ -204,10 +237,19 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
   }
 }
 
+// radix of the stage that starts at sub-length NS: the remainder radix
+// (L = R0 * 16^k) goes first, where it needs no twiddles
+template <int L, int NS> constexpr int stage_radix() {
+  if (NS != 1) return 16;
+  int r = 1, l = L;
+  while (l > 16) l /= 16;
+  r = l;  // L = r * 16^k with r in {2, 4, 8, 16}
+  return r;
+}
+
 template <typename T2, int L, int NS, bool INV>
 __device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf) {
-  constexpr int REM = L / NS;
-  constexpr int R = REM >= 16 ? 16 : REM;
+  constexpr int R = stage_radix<L, NS>();
   constexpr bool FIRST = NS == 1;
   constexpr bool LAST = NS * R == L;
   stockham_stage<T2, L, R, NS, FIRST, LAST, INV>(v, fib, q, twf);
@@ -319,12 +361,31 @@ __device__ __forceinline__ int64_t row_kbase(const Geom &g, int64_t rr) {
   return rr;
 }
 
-template <typename T2>
-__device__ __forceinline__ T2 inter_twiddle(const Geom &g, int64_t e) {
+// Inter-group twiddles of one thread's 16 elements: tw[m] = W_M^{(q + m*TPF) col},
+// M = L * S, as W_M^{q col} * (W_M^{TPF col})^m with the powers built like
+// twiddle_powers (five split-table reads instead of sixteen).
+template <typename T2, int L>
+__device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col, T2 (&tw)[16]) {
   const T2 *hi = (const T2 *)g.tw_hi;
   const T2 *lo = (const T2 *)g.tw_lo;
-  const int64_t m = (int64_t(1) << g.tw_sh) - 1;
-  return cmul(hi[e >> g.tw_sh], lo[e & m]);
+  const int64_t mask = (int64_t)L * g.S - 1;
+  constexpr int TPF = L / 16;
+  const int64_t eb = ((int64_t)TPF * col) & mask;
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((int64_t)q * col) & mask);
+  T2 b1 = split_twiddle(hi, lo, g.tw_sh, eb);
+  T2 b2 = split_twiddle(hi, lo, g.tw_sh, (2 * eb) & mask);
+  T2 b4 = split_twiddle(hi, lo, g.tw_sh, (4 * eb) & mask);
+  T2 b8 = split_twiddle(hi, lo, g.tw_sh, (8 * eb) & mask);
+  T2 p[8];
+  p[0].x = 1; p[0].y = 0;
+  p[1] = b1; p[2] = b2; p[3] = cmul(b2, b1);
+  p[4] = b4; p[5] = cmul(b4, b1); p[6] = cmul(b4, b2); p[7] = cmul(b4, p[3]);
+  const T2 q8 = cmul(wq, b8);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    tw[m] = cmul(wq, p[m]);
+    tw[m + 8] = cmul(q8, p[m]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -347,7 +408,7 @@ __global__ void __launch_bounds__(C *(L / 16))
   const int q = ROW ? t % TPF : t / C;
   T2 *fib = lds + c * FIB;
 
-  int64_t base, js, lo = 0, rr = 0;
+  int64_t base, js, col = 0, rr = 0;
   if (ROW) {
     rr = (int64_t)blockIdx.x * C + c;
     base = rr * L;
@@ -355,8 +416,8 @@ __global__ void __launch_bounds__(C *(L / 16))
   } else {
     const int64_t cpb = g.S / C;
     const int64_t hi = blockIdx.x / cpb;
-    lo = (blockIdx.x % cpb) * C + c;
-    base = hi * (int64_t)L * g.S + lo;
+    col = (blockIdx.x % cpb) * C + c;
+    base = hi * (int64_t)L * g.S + col;
     js = g.S;
   }
 
@@ -371,12 +432,12 @@ __global__ void __launch_bounds__(C *(L / 16))
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
 
+  T2 tw[16];
+  if (!ROW && (A.do_inv || A.do_fwd)) inter_twiddles<T2, L>(g, q, col, tw);
+
   if (!ROW && A.do_inv) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int64_t j = q + m * TPF;
-      v[m] = cmulc(v[m], inter_twiddle<T2>(g, j * lo));
-    }
+    for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
     stockham<T2, L, 1, true>(v, fib, q, twf);
   }
 
@@ -392,10 +453,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     stockham<T2, L, 1, false>(v, fib, q, twf);
     if (!ROW) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const int64_t j = q + m * TPF;
-        v[m] = cmul(v[m], inter_twiddle<T2>(g, j * lo));
-      }
+      for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
     }
   }
 

@@ -169,3 +169,16 @@ def test_plain_and_fused_agree(ctx, tmp_path):
     a = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     b = run_single_simulation(sd, 0, tmp_path, ctx=ctx, plain=True)
     assert rel_l2(a, b) < 1e-11
+
+
+def test_cfg2_full_step_count_error_budget(ctx, tmp_path):
+    """All 194 propagation steps of the Talbot-Lau chain (3 x 64 slices) at
+    reduced N: the complex64 + FP64-phase form stays within 1e-4 relative L2
+    of detector intensity against the complex128 oracle."""
+    from ravesim.multisim import run_single_simulation
+    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 18, slices=64, det_size=1e-4, dtype="c8"))
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    err = rel_l2(got, ref)
+    print(f"cfg2 N=2^18 194 steps c8 rel-L2 = {err:.3e}")
+    assert err < 1e-4
