@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--plain", action="store_true", help="one kernel per operation (baseline form)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the timed run, copy one field this many times (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every configuration of this path runs on a single "
@@ -150,6 +152,14 @@ def main():
         "detector_checksum": float(np.sum(out)),
     }
     print(json.dumps(line), flush=True)
+    if args.copy_probe:
+        a = ravesim.Field(ctx, N, 2e-9, dtype=dtype)
+        b = ravesim.Field(ctx, N, 2e-9, dtype=dtype)
+        for _ in range(args.copy_probe):
+            a.copy_to(b)
+        ctx.synchronize()
+        a.close()
+        b.close()
     ctx.close()
 
 

@@ -130,6 +130,10 @@ int rs_field_destroy(rs_field *f);
 int rs_field_upload(rs_field *f, const void *host);
 int rs_field_download(rs_field *f, void *host);
 
+/* dst <- src (same N and precision): big-wave Vector.copy_to
+ * (big-wave/vector.py:281-282), one copy kernel on the context stream. */
+int rs_field_copy(rs_field *dst, const rs_field *src);
+
 /* u = exp(-2 pi i r / wl) / sqrt(r), r = sqrt((x - x_source)^2 + z^2):
  * big-wave/propagation.py:141-151, fast-wave/fwcuda/kernels.cu:6-28. */
 int rs_point_source(rs_field *f, double wl, double x_source, double z);

@@ -10,6 +10,7 @@
 #include "../../include/ravesim.h"
 #include "engine.hpp"
 #include "npy.hpp"
+#include "rs_kernels.hpp"
 #include "simulation.hpp"
 
 struct rs_ctx {
@@ -260,6 +261,19 @@ int rs_field_download(rs_field *f, void *host) {
     rs::hip_check(hipMemcpyAsync(host, f->buf.p, f->buf.bytes, hipMemcpyDeviceToHost, f->owner->ctx->stream),
                   "download");
     rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
+  });
+}
+
+int rs_field_copy(rs_field *dst, const rs_field *src) {
+  if (!dst || !src) return fail(RS_ERR_INVALID, "NULL argument");
+  if (dst->grid.N != src->grid.N || dst->f64 != src->f64 || dst->owner != src->owner)
+    return fail(RS_ERR_INVALID, "fields differ in size, precision or context");
+  return guarded([&] {
+    hipStream_t st = dst->owner->ctx->stream;
+    rs::hip_check(dst->f64 ? rs::launch_copy<double>(dst->buf.p, src->buf.p, dst->grid.N, st)
+                           : rs::launch_copy<float>(dst->buf.p, src->buf.p, dst->grid.N, st),
+                  "copy");
+    rs::hip_check(hipStreamSynchronize(st), "sync");
   });
 }
 

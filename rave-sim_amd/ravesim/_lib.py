@@ -30,7 +30,7 @@ RS_MODE_PLAIN = 1
 EXPORTS = (
     "rs_version", "rs_last_error", "rs_ctx_create", "rs_ctx_destroy", "rs_ctx_synchronize",
     "rs_run_simdir", "rs_validate_simdir", "rs_field_create", "rs_field_destroy",
-    "rs_field_upload", "rs_field_download", "rs_point_source", "rs_propagate",
+    "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
     "rs_apply_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats",
 )
@@ -101,6 +101,7 @@ def _load() -> C.CDLL:
         "rs_field_destroy": (C.c_int, [P]),
         "rs_field_upload": (C.c_int, [P, P]),
         "rs_field_download": (C.c_int, [P, P]),
+        "rs_field_copy": (C.c_int, [P, P]),
         "rs_point_source": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
         "rs_propagate": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
         "rs_apply_grating": (C.c_int, [P, C.c_double, C.POINTER(rs_grating), C.c_int, C.c_double]),

@@ -139,6 +139,10 @@ class Field:
         check(lib.rs_field_download(self._h, a.ctypes.data_as(C.c_void_p)))
         return a
 
+    def copy_to(self, dst: "Field") -> None:
+        """dst <- self (big-wave Vector.copy_to)."""
+        check(lib.rs_field_copy(dst._h, self._h))
+
     def point_source(self, wl: float, x_source: float, z: float) -> None:
         check(lib.rs_point_source(self._h, wl, x_source, z))
 
