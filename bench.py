@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--dtype", default="c8", choices=["c8", "c16"])
     ap.add_argument("--plain", action="store_true", help="one kernel per operation (baseline form)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
@@ -97,6 +98,7 @@ def main():
     ctx = ravesim.Context(0)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet, dtype=dtype, plain=args.plain)
 
+    tuned = None if args.plain or args.no_tune else ctx.autotune(N, dtype)
     for _ in range(args.warmup):
         run_batch(ctx, N, 2e-9, members, **kw)
     ctx.synchronize()
@@ -144,7 +146,8 @@ def main():
         "config": {"workload": f"Talbot-Lau 3-grating chain, 46 keV, N=2^{args.lgN}, "
                                f"{args.slices} slices/grating, {steps_per_sim} propagation steps/sim, "
                                f"{'plain' if args.plain else 'fused'} form",
-                   "N": N, "steps_per_sim": steps_per_sim, "form": "plain" if args.plain else "fused"},
+                   "N": N, "steps_per_sim": steps_per_sim, "form": "plain" if args.plain else "fused",
+                   "split": tuned},
         "gsample_steps_per_s": round(gss, 3),
         "hbm_frac_whole_sim": round(gss * 1e9 * (32 if args.dtype == "c8" else 64) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,

@@ -216,6 +216,16 @@ typedef struct rs_pass_stat {
   double bytes;      /* summed algorithmic bytes (every field read + written once) */
 } rs_pass_stat;
 
+/* Autotuner (no counterpart in the reference): time every valid digit-group
+ * split of an N-point field on a synthetic grating slice and keep the fastest
+ * for this context.  out4 (optional) = {groups, lg top, lg middle, lg row};
+ * ms_per_slice (optional) = measured time of one slice with it. */
+int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int32_t *out4, double *ms_per_slice);
+/* Fix the digit-group split for (N, dtype) (lgR = log2 row length, lgA = log2
+ * top column length, -1 = default); persisted tuning results are re-applied
+ * this way. */
+int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA);
+
 /* Enable (1) / disable (0) per-launch event timing and reset the counters. */
 int rs_set_timing(rs_ctx *ctx, int enable);
 /* Copy min(n, 4) class records. */

@@ -377,6 +377,41 @@ int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bo
   });
 }
 
+int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int32_t *out4, double *ms_per_slice) {
+  if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  if (N < 64 || (N & (N - 1)) != 0 || N > (int64_t(1) << 30))
+    return fail(RS_ERR_UNSUPPORTED, "N must be a power of two in [2^6, 2^30]");
+  return guarded([&] {
+    const rs::Context::Tuned t = rs::autotune(*ctx->ctx, N, dtype == RS_C16);
+    rs::Plan &p = ctx->ctx->plan(N, dtype == RS_C16);
+    if (out4) {
+      out4[0] = p.ngroups;
+      out4[1] = p.lgA;
+      out4[2] = p.lgB;
+      out4[3] = p.lgR;
+    }
+    if (ms_per_slice) *ms_per_slice = t.ms;
+  });
+}
+
+int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
+  if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  return guarded([&] {
+    int lgN = 0;
+    while ((int64_t(1) << lgN) < N) ++lgN;
+    int ng, a, b, r;
+    rs::plan_split(lgN, dtype == RS_C16, lgR, lgA, &ng, &a, &b, &r);  // validates
+    auto &tv = ctx->ctx->tuned;
+    for (auto it = tv.begin(); it != tv.end(); ++it)
+      if (it->lgN == lgN && it->f64 == (dtype == RS_C16)) {
+        tv.erase(it);
+        break;
+      }
+    tv.push_back(rs::Context::Tuned{lgN, dtype == RS_C16, lgR, lgA, 0.0});
+  });
+}
+
 int rs_set_timing(rs_ctx *ctx, int enable) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
   ctx->ctx->timing = enable != 0;

@@ -163,7 +163,7 @@ Plan::~Plan() {
   for (void *q : lo) if (q) (void)hipFree(q);
 }
 
-void plan_split(int lgN, bool f64, int lgR_req, int *ng, int *lgA, int *lgB, int *lgR) {
+void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, int *lgB, int *lgR) {
   const int maxR = f64 ? kMaxRowLgF64 : kMaxRowLgF32;
   if (lgN < kMinRowLg || lgN > 30)
     throw Error(6, "N must be a power of two between 2^6 and 2^30");
@@ -176,13 +176,30 @@ void plan_split(int lgN, bool f64, int lgR_req, int *ng, int *lgA, int *lgB, int
   if (lgN - r < kMinColLg) r = lgN - kMinColLg;
   if (r < kMinRowLg) throw Error(6, "no digit-group split for this N");
   const int rest = lgN - r;
-  if (rest <= kMaxColLg) {
+  if (rest <= kMaxColLg && (lgA_req < 0 || lgA_req == rest)) {
     *ng = 2; *lgR = r; *lgA = rest; *lgB = 0;
     return;
   }
-  const int a = (rest + 1) / 2, b = rest - a;
-  if (a > kMaxColLg || b < kMinColLg) throw Error(6, "N too large for three digit groups");
+  int a = lgA_req >= 0 ? lgA_req : (rest + 1) / 2;
+  int b = rest - a;
+  if (a > kMaxColLg || a < kMinColLg || b < kMinColLg || b > kMaxColLg)
+    throw Error(6, "no three-group split for this N");
   *ng = 3; *lgR = r; *lgA = a; *lgB = b;
+}
+
+std::vector<std::pair<int, int>> plan_candidates(int lgN, bool f64) {
+  std::vector<std::pair<int, int>> out;
+  const int maxR = f64 ? kMaxRowLgF64 : kMaxRowLgF32;
+  if (lgN <= maxR) out.emplace_back(lgN, lgN);
+  for (int r = kMinRowLg; r <= std::min(maxR, lgN - kMinColLg); ++r) {
+    const int rest = lgN - r;
+    if (rest <= kMaxColLg) out.emplace_back(r, rest);
+    for (int a = kMinColLg; a <= kMaxColLg; ++a) {
+      const int b = rest - a;
+      if (b >= kMinColLg && b <= kMaxColLg) out.emplace_back(r, a);
+    }
+  }
+  return out;
 }
 
 template <typename S>
@@ -212,7 +229,7 @@ static void *make_twiddles(int lgM, int64_t count, int64_t mult, bool f64) {
   return f64 ? upload_table<double>(re, im) : upload_table<float>(re, im);
 }
 
-std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req) {
+std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
   int lgN = 0;
   while ((int64_t(1) << lgN) < N) ++lgN;
   if ((int64_t(1) << lgN) != N) throw Error(6, "N must be a power of two");
@@ -220,7 +237,7 @@ std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req) {
   p->N = N;
   p->lgN = lgN;
   p->f64 = f64;
-  plan_split(lgN, f64, lgR_req, &p->ngroups, &p->lgA, &p->lgB, &p->lgR);
+  plan_split(lgN, f64, lgR_req, lgA_req, &p->ngroups, &p->lgA, &p->lgB, &p->lgR);
   // in-fiber tables: row, top, middle
   p->fib[0] = make_twiddles(p->lgR, int64_t(1) << p->lgR, 1, f64);
   if (p->ngroups >= 2) p->fib[1] = make_twiddles(p->lgA, int64_t(1) << p->lgA, 1, f64);
@@ -285,11 +302,26 @@ Context::~Context() {
   plans_.clear();
   if (stream) (void)hipStreamDestroy(stream);
 }
-Plan &Context::plan(int64_t N, bool f64, int lgR) {
+const Context::Tuned *Context::find_tuned(int lgN, bool f64) const {
+  for (const auto &t : tuned)
+    if (t.lgN == lgN && t.f64 == f64) return &t;
+  return nullptr;
+}
+
+Plan &Context::plan(int64_t N, bool f64, int lgR, int lgA) {
+  if (lgR < 0 && lgA < 0) {
+    int lgN = 0;
+    while ((int64_t(1) << lgN) < N) ++lgN;
+    if (const Tuned *t = find_tuned(lgN, f64)) {
+      lgR = t->lgR;
+      lgA = t->lgA;
+    }
+  }
   for (size_t i = 0; i < plans_.size(); ++i)
-    if (plans_[i]->N == N && plans_[i]->f64 == f64 && plan_lgR_[i] == lgR) return *plans_[i];
-  plans_.push_back(make_plan(N, f64, lgR));
-  plan_lgR_.push_back(lgR);
+    if (plans_[i]->N == N && plans_[i]->f64 == f64 && plan_req_[i] == std::make_pair(lgR, lgA))
+      return *plans_[i];
+  plans_.push_back(make_plan(N, f64, lgR, lgA));
+  plan_req_.emplace_back(lgR, lgA);
   return *plans_.back();
 }
 void Context::sync() {
@@ -808,7 +840,7 @@ template <typename S> struct Exec {
 template <typename S>
 std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   hip_check(hipSetDevice(ctx.device), "hipSetDevice");
-  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR);
+  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA);
   const int64_t npx = detector_pixels(rc.det_size, rc.det_px);
   const std::vector<int64_t> hb = detector_bounds(rc.N, rc.dx, rc.det_size, rc.det_px);
   DevBuf dbounds(hb.size() * sizeof(int64_t));
@@ -879,6 +911,75 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
 std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   if (members.empty()) return {};
   return rc.f64 ? run_members_t<double>(ctx, rc, members) : run_members_t<float>(ctx, rc, members);
+}
+
+Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps) {
+  int lgN = 0;
+  while ((int64_t(1) << lgN) < N) ++lgN;
+  // synthetic slice loop: plane wave, `reps` grating slices of a 4.2 um
+  // grating at 46 keV, 1 mm detector
+  const double dx = 2e-9, wl = energy_to_wavelength(46000.0);
+  Member m;
+  m.wl = wl;
+  Op src{OpKind::Source};
+  src.x.kind = XOP_ONES;
+  src.x.post[0] = 1.0;
+  m.ops.push_back(src);
+  for (int i = 0; i < reps + 1; ++i) {
+    Op mk{OpKind::Mask};
+    mk.x.kind = XOP_MASK;
+    mk.x.fa[0] = 0.9;
+    mk.x.fa[1] = 0.1;
+    mk.x.fb[0] = 1.0;
+    mk.x.post[0] = 1.0;
+    mk.x.pitch = 4.2e-6;
+    mk.x.inv_pitch = 1.0 / 4.2e-6;
+    mk.x.dc = 0.5;
+    mk.x.half_dc = 0.25;
+    mk.x.dx = dx;
+    mk.x.chunk = N;
+    mk.x.single_chunk = 1;
+    mk.x.x0 = (double)(-(N / 2)) * dx;
+    mk.x.xstep = dx;
+    m.ops.push_back(mk);
+    Op p{OpKind::Prop};
+    p.dz = 1e-6;
+    p.fc = 1e30;
+    m.ops.push_back(p);
+  }
+  m.ops.push_back(Op{OpKind::Flush});
+  RunConfig rc;
+  rc.N = N;
+  rc.dx = dx;
+  rc.f64 = f64;
+  rc.n_out = 1;
+  const bool was = ctx.timing;
+  Context::Tuned best{lgN, f64, -1, -1, 1e300};
+  for (auto cand : plan_candidates(lgN, f64)) {
+    rc.lgR = cand.first;
+    rc.lgA = cand.second;
+    std::vector<Member> ms{m};
+    ctx.timing = false;
+    (void)run_members(ctx, rc, ms);  // warm: plan tables, code objects
+    ms = {m};
+    for (auto &st : ctx.stats) st = PassStat{};
+    ctx.timing = true;
+    (void)run_members(ctx, rc, ms);
+    double tot = 0;
+    for (auto &st : ctx.stats) tot += st.ms;
+    const double per = tot / (reps + 1);
+    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per};
+  }
+  ctx.timing = was;
+  for (auto &st : ctx.stats) st = PassStat{};
+  bool replaced = false;
+  for (auto &t : ctx.tuned)
+    if (t.lgN == lgN && t.f64 == f64) {
+      t = best;
+      replaced = true;
+    }
+  if (!replaced) ctx.tuned.push_back(best);
+  return best;
 }
 
 }  // namespace rs

@@ -76,9 +76,12 @@ struct Plan {
   std::string describe() const;
 };
 
-// default split of lgN into digit groups (override with lgR >= 0)
-void plan_split(int lgN, bool f64, int lgR_req, int *ng, int *lgA, int *lgB, int *lgR);
-std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req = -1);
+// split of lgN into digit groups; lgR_req / lgA_req >= 0 override the default
+// (row as long as the kernels allow, remaining bits halved between columns)
+void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, int *lgB, int *lgR);
+std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req = -1, int lgA_req = -1);
+// every valid (lgR, lgA) split of lgN, the autotuner's candidates
+std::vector<std::pair<int, int>> plan_candidates(int lgN, bool f64);
 
 // ----------------------------------------------------------------------------
 // member programs
@@ -112,7 +115,8 @@ struct RunConfig {
   int n_out = 1;             // accumulator rows
   int max_resident = 0;      // 0 = all members at once
   bool use_graph = false;
-  int lgR = -1;              // row length override (autotuner)
+  int lgR = -1;              // row length override (autotuner), -1 = tuned or default
+  int lgA = -1;              // top column length override
   void *const *fields = nullptr;  // caller-owned field buffers (one per member), else allocated
 };
 
@@ -141,13 +145,27 @@ class Context {
   hipStream_t stream = nullptr;
   bool timing = false;
   PassStat stats[PC_COUNT];
-  Plan &plan(int64_t N, bool f64, int lgR = -1);
+  // plan for (N, precision); explicit lgR/lgA, else the tuned split, else default
+  Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1);
   void sync();
+  // autotuner results: (lgN, f64) -> (lgR, lgA, ms per slice)
+  struct Tuned {
+    int lgN;
+    bool f64;
+    int lgR, lgA;
+    double ms;
+  };
+  std::vector<Tuned> tuned;
+  const Tuned *find_tuned(int lgN, bool f64) const;
 
  private:
   std::vector<std::unique_ptr<Plan>> plans_;
-  std::vector<int> plan_lgR_;
+  std::vector<std::pair<int, int>> plan_req_;
 };
+
+// Time every candidate split of N on a synthetic grating slice (mask +
+// propagation) and record the fastest in ctx.tuned.  Returns it.
+Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3);
 
 // device buffer RAII
 struct DevBuf {

@@ -272,20 +272,31 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
   bool mul = true;
   switch (op.kind) {
     case XOP_MASK: {
-      int64_t c = 0, k = n, len = N;
-      if (!op.single_chunk) {
-        c = n / op.chunk;
-        k = n - c * op.chunk;
-        len = N - c * op.chunk;
+      double x;
+      if (op.single_chunk) {
+        x = (double)n * op.xstep + op.x0;
+      } else {
+        const int64_t c = n / op.chunk;
+        const int64_t k = n - c * op.chunk;
+        int64_t len = N - c * op.chunk;
         if (len > op.chunk) len = op.chunk;
+        const int64_t start = c * op.chunk - N / 2;
+        const double x_start = (double)start * op.dx + op.xpos;
+        const double x_stop = x_start + (double)len * op.dx;
+        const double step = (x_stop - x_start) / (double)len;
+        x = (double)k * step + x_start;
       }
-      const int64_t start = c * op.chunk - N / 2;
-      const double x_start = (double)start * op.dx + op.xpos;
-      const double x_stop = x_start + (double)len * op.dx;
-      const double step = (x_stop - x_start) / (double)len;
-      const double x = (double)k * step + x_start;
-      const double t = x / op.pitch + op.half_dc;
-      const double ph = t - floor(t);
+      // t = x/pitch + dc/2 as big-wave rounds it.  x * (1/pitch) differs from
+      // the correctly rounded quotient by a few ulp of t; outside a guard band
+      // around the mask edges (ph = 0 and ph = dc) that cannot change the
+      // decision, inside it the exact quotient is used.
+      double t = x * op.inv_pitch + op.half_dc;
+      double ph = t - floor(t);
+      const double guard = fabs(t) * 1e-13 + 1e-300;
+      if (ph < guard || ph > 1.0 - guard || fabs(ph - op.dc) < guard) {
+        t = x / op.pitch + op.half_dc;
+        ph = t - floor(t);
+      }
       if (ph < op.dc) { re = op.fa[0]; im = op.fa[1]; }
       else { re = op.fb[0]; im = op.fb[1]; }
       break;

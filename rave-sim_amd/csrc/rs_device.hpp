@@ -34,6 +34,8 @@ struct XOp {
   double post[2];
   // grating (big-wave/optical_element.py:341-357)
   double pitch, dc, half_dc, xpos, dx;
+  double inv_pitch;       // 1/pitch for the fast mask test (exact division near edges)
+  double x0, xstep;       // single chunk: linspace start and step, computed on the host
   int64_t chunk;
   // point source (big-wave/propagation.py:141-151)
   double xs, z, kphase;

@@ -176,6 +176,13 @@ XOp mask_op(const SimConfig &c, const GratingCfg &g, int slice, int step, const 
   x.dx = c.dx;
   x.chunk = c.chunk_size;
   x.single_chunk = c.N <= c.chunk_size;
+  x.inv_pitch = 1.0 / g.pitch;
+  // np.linspace(x_start, x_start + N dx, N, endpoint=False) of the single chunk
+  // (big-wave/optical_element.py:352-355), evaluated as numpy does
+  const double x_start = (double)(-(c.N / 2)) * c.dx + x.xpos;
+  const double x_stop = x_start + (double)c.N * c.dx;
+  x.x0 = x_start;
+  x.xstep = (x_stop - x_start) / (double)c.N;
   return x;
 }
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "rs_run_simdir", "rs_validate_simdir", "rs_field_create", "rs_field_destroy",
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
     "rs_apply_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
-    "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats",
+    "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
 )
 
 
@@ -113,6 +113,8 @@ def _load() -> C.CDLL:
                                          C.POINTER(C.c_int64), C.c_int64]),
         "rs_batch_run": (C.c_int, [P, C.POINTER(rs_batch_desc), C.POINTER(C.c_double), C.c_int64]),
         "rs_set_timing": (C.c_int, [P, C.c_int]),
+        "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+        "rs_set_split": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.c_int]),
         "rs_get_pass_stats": (C.c_int, [P, C.POINTER(rs_pass_stat), C.c_int]),
     }
     for name, (res, args) in sig.items():

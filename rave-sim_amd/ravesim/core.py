@@ -42,6 +42,18 @@ class Context:
         """Per-launch HIP-event timing of the fused passes (resets counters)."""
         check(lib.rs_set_timing(self.handle, 1 if enable else 0))
 
+    def autotune(self, N: int, dtype=np.complex64) -> dict:
+        """Pick the fastest digit-group split for (N, dtype) on this device."""
+        out = (C.c_int32 * 4)()
+        ms = C.c_double()
+        code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16
+        check(lib.rs_autotune(self.handle, int(N), code, out, C.byref(ms)))
+        return {"groups": out[0], "lgA": out[1], "lgB": out[2], "lgR": out[3], "ms_per_slice": ms.value}
+
+    def set_split(self, N: int, dtype, lgR: int, lgA: int = -1) -> None:
+        code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16
+        check(lib.rs_set_split(self.handle, int(N), code, int(lgR), int(lgA)))
+
     def pass_stats(self) -> dict:
         """{class: (launches, total_ms, algorithmic bytes)} for row / top / mid / other."""
         arr = (_lib.rs_pass_stat * 4)()
