@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--plain", action="store_true", help="one kernel per operation (baseline form)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
+    ap.add_argument("--split", default="", help="fixed digit-group split 'lgR,lgA' instead of tuning")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
@@ -98,7 +99,12 @@ def main():
     ctx = ravesim.Context(0)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet, dtype=dtype, plain=args.plain)
 
-    tuned = None if args.plain or args.no_tune else ctx.autotune(N, dtype)
+    if args.split:
+        lgR, lgA = (int(v) for v in args.split.split(","))
+        ctx.set_split(N, dtype, lgR, lgA)
+        tuned = {"lgR": lgR, "lgA": lgA, "fixed": True}
+    else:
+        tuned = None if args.plain or args.no_tune else ctx.autotune(N, dtype)
     for _ in range(args.warmup):
         run_batch(ctx, N, 2e-9, members, **kw)
     ctx.synchronize()

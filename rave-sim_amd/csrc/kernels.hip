@@ -408,7 +408,9 @@ __global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
   constexpr int TPF = L / 16;
-  constexpr int FIB = L + L / 16;
+  // fiber slab stride: odd, so that lanes working on neighbouring fibers at
+  // the same offset (column passes) hit different LDS banks
+  constexpr int FIB = (L + L / 16) | 1;
   __shared__ T2 lds[C * FIB];
 
   const FieldArgs &A = args[blockIdx.y];
