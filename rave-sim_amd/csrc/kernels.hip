@@ -333,12 +333,40 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
   return mul ? cmul(v, f) : f;
 }
 
-template <typename S> __device__ __forceinline__ void sincospi_t(S x, S *s, S *c);
-template <> __device__ __forceinline__ void sincospi_t<float>(float x, float *s, float *c) {
-  sincospif(x, s, c);
+// (cos, sin)(2 pi t) for t in [-1/2, 1/2] given in FP64.  The quadrant is
+// split off in FP64 (exact: t4 = 4t, q = rint(t4), f = t4 - q in [-1/2, 1/2]),
+// then x = f pi/2 in [-pi/4, pi/4] goes through the Taylor series of sin
+// (to x^9, truncation < 2e-9) and cos (to x^10, < 2e-10) in the field
+// precision; the quadrant rotates (cos, sin) by q * 90 degrees.
+template <typename S> __device__ __forceinline__ void cis_rev(double t, S *c_out, S *s_out);
+
+template <> __device__ __forceinline__ void cis_rev<float>(double t, float *c_out, float *s_out) {
+  const double t4 = 4.0 * t;
+  const double q = rint(t4);
+  const float x = (float)((t4 - q) * 1.5707963267948966);
+  const float x2 = x * x;
+  float sp = -1.0f / 362880.0f;
+  sp = fmaf(sp, x2, 1.0f / 5040.0f);
+  sp = fmaf(sp, x2, -1.0f / 120.0f);
+  sp = fmaf(sp, x2, 1.0f / 6.0f);
+  // sin x = x - x^3 (1/6 - x^2/120 + ...)
+  const float sn = fmaf(-x * x2, sp, x);
+  float cp = -1.0f / 3628800.0f;
+  cp = fmaf(cp, x2, 1.0f / 40320.0f);
+  cp = fmaf(cp, x2, -1.0f / 720.0f);
+  cp = fmaf(cp, x2, 1.0f / 24.0f);
+  cp = fmaf(cp, x2, -0.5f);
+  const float cs = fmaf(cp, x2, 1.0f);
+  const int qi = (int)q & 3;
+  // rotate by q quarter turns: (c, s) -> (-s, c) per turn
+  const float c1 = (qi & 1) ? -sn : cs;
+  const float s1 = (qi & 1) ? cs : sn;
+  *c_out = (qi & 2) ? -c1 : c1;
+  *s_out = (qi & 2) ? -s1 : s1;
 }
-template <> __device__ __forceinline__ void sincospi_t<double>(double x, double *s, double *c) {
-  sincospi(x, s, c);
+
+template <> __device__ __forceinline__ void cis_rev<double>(double t, double *c_out, double *s_out) {
+  sincospi(2.0 * t, s_out, c_out);
 }
 
 // spectrum multiply at FFT bin k: phase in revolutions computed in FP64 and
@@ -355,7 +383,7 @@ __device__ __forceinline__ T2 apply_kop(const KOp &op, int64_t k, int64_t N, T2 
   double t = op.alpha_rev * (kd * kd) + op.beta_rev;
   t -= rint(t);
   S s, c;
-  sincospi_t<S>((S)(2.0 * t), &s, &c);
+  cis_rev<S>(t, &c, &s);
   T2 h;
   h.x = c * (S)op.scale[0] - s * (S)op.scale[1];
   h.y = c * (S)op.scale[1] + s * (S)op.scale[0];
@@ -404,7 +432,7 @@ __device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col
 // ---------------------------------------------------------------------------
 
 template <typename S, int L, int C, bool ROW>
-__global__ void __launch_bounds__(C *(L / 16))
+__global__ void __launch_bounds__(C *(L / 16)) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? 4 : 2)))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
   constexpr int TPF = L / 16;
