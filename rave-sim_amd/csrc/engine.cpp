@@ -180,9 +180,14 @@ void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, 
     *ng = 2; *lgR = r; *lgA = rest; *lgB = 0;
     return;
   }
+  const int maxW = f64 ? kMaxWideColLgF64 : kMaxWideColLgF32;
+  if (lgA_req >= 0 && lgA_req == rest && rest <= maxW) {  // wide two-group split
+    *ng = 2; *lgR = r; *lgA = rest; *lgB = 0;
+    return;
+  }
   int a = lgA_req >= 0 ? lgA_req : (rest + 1) / 2;
   int b = rest - a;
-  if (a > kMaxColLg || a < kMinColLg || b < kMinColLg || b > kMaxColLg)
+  if (a > maxW || a < kMinColLg || b < kMinColLg || b > maxW)
     throw Error(6, "no three-group split for this N");
   *ng = 3; *lgR = r; *lgA = a; *lgB = b;
 }
@@ -191,9 +196,10 @@ std::vector<std::pair<int, int>> plan_candidates(int lgN, bool f64) {
   std::vector<std::pair<int, int>> out;
   const int maxR = f64 ? kMaxRowLgF64 : kMaxRowLgF32;
   if (lgN <= maxR) out.emplace_back(lgN, lgN);
+  const int maxW = f64 ? kMaxWideColLgF64 : kMaxWideColLgF32;
   for (int r = kMinRowLg; r <= std::min(maxR, lgN - kMinColLg); ++r) {
     const int rest = lgN - r;
-    if (rest <= kMaxColLg) out.emplace_back(r, rest);
+    if (rest <= maxW) out.emplace_back(r, rest);
     for (int a = kMinColLg; a <= kMaxColLg; ++a) {
       const int b = rest - a;
       if (b >= kMinColLg && b <= kMaxColLg) out.emplace_back(r, a);
@@ -255,6 +261,7 @@ std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
 
 Geom Plan::geom_row() const {
   Geom g{};
+  g.xcd_segs = 1;
   g.N = N;
   g.S = 1;
   g.top = ngroups == 1;
@@ -265,9 +272,15 @@ Geom Plan::geom_row() const {
   g.tw_fib = fib[0];
   return g;
 }
+static int32_t line_sharing(int lgL, bool f64) {
+  const int bytes = col_fibers(1 << lgL, f64) * (f64 ? 16 : 8);
+  return bytes >= 128 ? 1 : 128 / bytes;
+}
+
 Geom Plan::geom_top() const {
   Geom g = geom_row();
   g.top = 1;
+  g.xcd_segs = line_sharing(lgA, f64);
   g.S = int64_t(1) << (lgB + lgR);
   g.tw_fib = fib[1];
   g.tw_sh = sh[0];
@@ -278,6 +291,7 @@ Geom Plan::geom_top() const {
 Geom Plan::geom_mid() const {
   Geom g = geom_row();
   g.top = 0;
+  g.xcd_segs = line_sharing(lgB, f64);
   g.S = int64_t(1) << lgR;
   g.tw_fib = fib[2];
   g.tw_sh = sh[1];

@@ -432,7 +432,7 @@ __device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col
 // ---------------------------------------------------------------------------
 
 template <typename S, int L, int C, bool ROW>
-__global__ void __launch_bounds__(C *(L / 16)) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 ? 4 : 2)))
+__global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
   constexpr int TPF = L / 16;
@@ -455,9 +455,21 @@ __global__ void __launch_bounds__(C *(L / 16)) __attribute__((amdgpu_waves_per_e
     base = rr * L;
     js = 1;
   } else {
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; when a
+    // row segment of C columns is shorter than a 128 B line, the `segs`
+    // workgroups that share one line are given ids on the same XCD so that
+    // its L2 merges their partial-line reads and writes
+    int64_t bid = blockIdx.x;
+    if (g.xcd_segs > 1) {
+      const int64_t G = 8 * (int64_t)g.xcd_segs;
+      if (bid < ((int64_t)gridDim.x / G) * G) {
+        const int64_t r = bid % G;
+        bid = (bid - r) + (r % 8) * g.xcd_segs + r / 8;
+      }
+    }
     const int64_t cpb = g.S / C;
-    const int64_t hi = blockIdx.x / cpb;
-    col = (blockIdx.x % cpb) * C + c;
+    const int64_t hi = bid / cpb;
+    col = (bid % cpb) * C + c;
     base = hi * (int64_t)L * g.S + col;
     js = g.S;
   }
@@ -473,10 +485,9 @@ __global__ void __launch_bounds__(C *(L / 16)) __attribute__((amdgpu_waves_per_e
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
 
-  T2 tw[16];
-  if (!ROW && (A.do_inv || A.do_fwd)) inter_twiddles<T2, L>(g, q, col, tw);
-
   if (!ROW && A.do_inv) {
+    T2 tw[16];
+    inter_twiddles<T2, L>(g, q, col, tw);
 #pragma unroll
     for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
     stockham<T2, L, 1, true>(v, fib, q, twf);
@@ -493,6 +504,8 @@ __global__ void __launch_bounds__(C *(L / 16)) __attribute__((amdgpu_waves_per_e
   if (A.do_fwd) {
     stockham<T2, L, 1, false>(v, fib, q, twf);
     if (!ROW) {
+      T2 tw[16];  // recomputed rather than kept live across the FFTs (registers)
+      inter_twiddles<T2, L>(g, q, col, tw);
 #pragma unroll
       for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
     }
@@ -606,13 +619,9 @@ inline unsigned grid_stride_blocks(int64_t n) {
 
 // columns per workgroup for a column pass of fiber length L: 16 columns of
 // complex64 (128 B per row segment) / 8 of complex128, more for short fibers
-template <typename S> constexpr int col_C(int L) {
-  return L <= 16 ? (sizeof(S) == 4 ? 64 : 32) : L <= 64 ? (sizeof(S) == 4 ? 32 : 16) : (sizeof(S) == 4 ? 16 : 8);
-}
-
 template <typename S, int L>
 static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
-  constexpr int C = col_C<S>(L);
+  constexpr int C = col_fibers(L, sizeof(S) == 8);
   if (g.S % C != 0) return hipErrorInvalidValue;
   const int64_t nblk = g.N / ((int64_t)L * C);
   dim3 grid((unsigned)nblk, (unsigned)nf);
@@ -654,6 +663,11 @@ hipError_t launch_fiber_pass(int lgL, bool row, const FieldArgs *args, int nf, c
     case 7: return launch_col<S, 128>(args, nf, g, st);
     case 8: return launch_col<S, 256>(args, nf, g, st);
     case 9: return launch_col<S, 512>(args, nf, g, st);
+    case 10: return launch_col<S, 1024>(args, nf, g, st);
+    case 11: return launch_col<S, 2048>(args, nf, g, st);
+    case 12: return launch_col<S, 4096>(args, nf, g, st);
+    case 13: if constexpr (sizeof(S) == 4) return launch_col<S, 8192>(args, nf, g, st);
+             else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

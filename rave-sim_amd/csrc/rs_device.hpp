@@ -77,7 +77,7 @@ struct Geom {
   int32_t lgA, lgB, lgR;  // log2 of top / middle / row group sizes (lgB = 0 if 2 groups)
   // inter-group twiddle W_M^e, e = k*lo, M = L*S, split as hi[e >> sh] * lo[e & mask]
   int32_t tw_sh;
-  int32_t pad_;
+  int32_t xcd_segs;  // column passes: workgroups sharing one 128 B line (1 = none)
   const void *tw_hi;
   const void *tw_lo;
   const void *tw_fib;  // W_L^j, j < L

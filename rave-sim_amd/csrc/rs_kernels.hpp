@@ -25,6 +25,17 @@ constexpr int kMinRowLg = 6;
 constexpr int kMaxRowLgF32 = 13;
 constexpr int kMaxRowLgF64 = 12;
 constexpr int kMinColLg = 4;
-constexpr int kMaxColLg = 9;
+constexpr int kMaxColLg = 9;        // default plans: columns of up to 512 with full 128 B segments
+constexpr int kMaxWideColLgF32 = 13;  // wide column passes (C * L = 16384 complex64, 128 KB LDS)
+constexpr int kMaxWideColLgF64 = 12;  // (C * L = 8192 complex128)
+
+// fibers per workgroup of a column pass of length L: 128 B row segments up to
+// L = 512 (more fibers for short L), then C * L fixed at 128 KB of LDS
+constexpr int col_fibers(int L, bool f64) {
+  if (L >= 1024) return (f64 ? 8192 : 16384) / L;
+  if (L <= 16) return f64 ? 32 : 64;
+  if (L <= 64) return f64 ? 16 : 32;
+  return f64 ? 8 : 16;
+}
 
 }  // namespace rs

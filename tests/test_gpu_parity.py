@@ -182,3 +182,31 @@ def test_cfg2_full_step_count_error_budget(ctx, tmp_path):
     err = rel_l2(got, ref)
     print(f"cfg2 N=2^18 194 steps c8 rel-L2 = {err:.3e}")
     assert err < 1e-4
+
+
+# digit-group splits forced through rs_set_split: wide column passes
+# (C * L = 16384 complex64 / 8192 complex128 per workgroup, XCD-grouped
+# partial-line segments) and three-group splits with uneven columns
+SPLIT_CASES = [(2 ** 20, np.complex64, 10, 10), (2 ** 22, np.complex64, 9, 13),
+               (2 ** 24, np.complex64, 12, 12), (2 ** 20, np.complex128, 8, 12),
+               (2 ** 21, np.complex128, 10, 11), (2 ** 22, np.complex64, 8, 5)]
+
+
+@pytest.mark.parametrize("N,dtype,lgR,lgA", SPLIT_CASES)
+def test_forced_splits_match_oracle(N, dtype, lgR, lgA):
+    import ravesim
+    c = ravesim.Context(0)
+    c.set_split(N, dtype, lgR, lgA)
+    dx, wl, dz = 2e-9, 2.7e-11, 1.7e-3
+    u = rand_field(N, lgR * 100 + lgA)
+    fc = 0.3 / dx
+    f = ravesim.Field(c, N, dx, dtype=dtype)
+    f.upload(u)
+    f.propagate(wl, dz, fc)
+    got = f.download()
+    ref = bw.Fields(u.astype(np.complex128))
+    bw.propagate(ref, dx, wl, dz, N, fc)
+    tol = 1e-10 if dtype == np.complex128 else 2e-6
+    assert rel_l2(got, ref.u) < tol
+    f.close()
+    c.close()
