@@ -204,6 +204,50 @@ typedef struct rs_batch_desc {
 int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *desc, double *out, int64_t n_out);
 
 /* ------------------------------------------------------------------ */
+/* serving (no counterpart in the reference, which runs one process per */
+/* source point: fast-wave/Readme.md:21-34)                             */
+/* ------------------------------------------------------------------ */
+
+typedef struct rs_server rs_server;
+
+/* One request: a setup at nr_energies energies (summed incoherently with
+ * weights); gratings may carry several x_positions (phase stepping), giving
+ * n_steps detector rows.  gratings holds nr_energies * nr_gratings entries
+ * (delta/beta depend on the energy), energy-major. */
+typedef struct rs_request {
+  rs_grid grid;
+  rs_detector det;
+  double z_detector;
+  int32_t dtype;            /* RS_C8 or RS_C16 */
+  int32_t nr_gratings;
+  int32_t nr_energies;
+  int32_t source_kind;      /* 0 point source, 1 plane wave */
+  const double *energies_ev;
+  const double *weights;    /* NULL = all 1 */
+  const rs_grating *gratings;
+  const double *cutoff_angles;  /* nr_gratings + 1 */
+  double source_x, source_z;
+} rs_request;
+
+/* Start a scheduler thread on the context's device.  reserve_bytes of free
+ * device memory are left unused (<= 0: 4 GiB); max_jobs bounds the number of
+ * (request, energy) jobs resident at once (<= 0: no bound). */
+int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out);
+/* Queue a request (all inputs copied); returns immediately. */
+int rs_server_submit(rs_server *srv, const rs_request *req, int64_t *id);
+/* state: 0 queued, 1 running, 2 done, 3 failed.  When done, out receives the
+ * (n_steps, n_px) rows if out_cap suffices; shape2 = {n_steps, n_px};
+ * times3 = {queue wait, service time, latency} in seconds. */
+int rs_server_poll(rs_server *srv, int64_t id, int32_t *state, double *out, int64_t out_cap,
+                   int64_t *shape2, double *times3);
+/* Block until request id is done or timeout_s passes (1 = done, 0 = timeout). */
+int rs_server_wait(rs_server *srv, int64_t id, double timeout_s, int32_t *done);
+/* stats6 = {ticks, launches, jobs per launch (mean), jobs done, max resident jobs, busy s} */
+int rs_server_stats(rs_server *srv, double *stats6);
+/* Finish all queued work and stop the scheduler. */
+int rs_server_destroy(rs_server *srv);
+
+/* ------------------------------------------------------------------ */
 /* measurement (no counterpart in the reference; fast-wave logs        */
 /* per-element cudaEvent timings, fast-wave/src/simulation.cpp:13-32)  */
 /* ------------------------------------------------------------------ */

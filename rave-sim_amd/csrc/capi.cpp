@@ -12,9 +12,14 @@
 #include "npy.hpp"
 #include "rs_kernels.hpp"
 #include "simulation.hpp"
+#include "serve.hpp"
 
 struct rs_ctx {
   std::unique_ptr<rs::Context> ctx;
+};
+
+struct rs_server {
+  std::unique_ptr<rs::Server> srv;
 };
 
 struct rs_field {
@@ -409,6 +414,98 @@ int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
         break;
       }
     tv.push_back(rs::Context::Tuned{lgN, dtype == RS_C16, lgR, lgA, 0.0});
+  });
+}
+
+int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out) {
+  if (!ctx || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    auto s = std::make_unique<rs_server>();
+    s->srv = std::make_unique<rs::Server>(*ctx->ctx, reserve_bytes, max_jobs);
+    *out = s.release();
+  });
+}
+
+int rs_server_submit(rs_server *srv, const rs_request *q, int64_t *id) {
+  if (!srv || !q || !id) return fail(RS_ERR_INVALID, "NULL argument");
+  if (q->nr_energies < 1 || !q->energies_ev || (q->nr_gratings > 0 && !q->gratings) || !q->cutoff_angles)
+    return fail(RS_ERR_INVALID, "request needs energies, gratings and cutoff angles");
+  if (q->dtype != RS_C8 && q->dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  return guarded([&] {
+    rs::ServeRequest r;
+    for (int e = 0; e < q->nr_energies; ++e) {
+      rs::SimConfig c;
+      c.N = q->grid.N;
+      c.dx = q->grid.dx;
+      c.chunk_size = q->grid.chunk_size > 0 ? q->grid.chunk_size : q->grid.N;
+      c.f64 = q->dtype == RS_C16;
+      c.z_detector = q->z_detector;
+      c.detector_size = q->det.size;
+      c.px = q->det.pixel_x;
+      c.py = q->det.pixel_y;
+      c.energy = q->energies_ev[e];
+      c.wl = rs::energy_to_wavelength(c.energy);
+      c.source.kind = q->source_kind == 1 ? rs::SourceCfg::Plane : rs::SourceCfg::Point;
+      c.source.x = q->source_x;
+      c.source.z = q->source_z;
+      for (int k = 0; k < q->nr_gratings; ++k)
+        c.elements.push_back(grating_from(&q->gratings[(size_t)e * q->nr_gratings + k]));
+      for (int k = 0; k <= q->nr_gratings; ++k) c.cutoff_angles.push_back(q->cutoff_angles[k]);
+      r.members.push_back(std::move(c));
+      r.weights.push_back(q->weights ? q->weights[e] : 1.0);
+    }
+    *id = srv->srv->submit(std::move(r));
+  });
+}
+
+int rs_server_poll(rs_server *srv, int64_t id, int32_t *state, double *out, int64_t out_cap, int64_t *shape2,
+                   double *times3) {
+  if (!srv || !state) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    const rs::ServeResult r = srv->srv->poll(id);
+    *state = r.state;
+    if (shape2) {
+      shape2[0] = r.n_steps;
+      shape2[1] = r.n_px;
+    }
+    if (r.state == 3) throw rs::Error(RS_ERR_DEVICE, "request failed: " + r.error);
+    if (r.state == 2) {
+      if (out && out_cap >= (int64_t)r.rows.size()) std::memcpy(out, r.rows.data(), r.rows.size() * sizeof(double));
+      if (times3) {
+        times3[0] = r.start_t - r.submit_t;
+        times3[1] = r.done_t - r.start_t;
+        times3[2] = r.done_t - r.submit_t;
+      }
+    }
+  });
+}
+
+int rs_server_wait(rs_server *srv, int64_t id, double timeout_s, int32_t *done) {
+  if (!srv) return fail(RS_ERR_INVALID, "server is NULL");
+  return guarded([&] {
+    const bool d = srv->srv->wait(id, timeout_s);
+    if (done) *done = d ? 1 : 0;
+  });
+}
+
+int rs_server_stats(rs_server *srv, double *o) {
+  if (!srv || !o) return fail(RS_ERR_INVALID, "NULL argument");
+  return guarded([&] {
+    const rs::ServeStats s = srv->srv->stats();
+    o[0] = (double)s.ticks;
+    o[1] = (double)s.launches;
+    o[2] = s.launches ? (double)s.merged_jobs / (double)s.launches : 0.0;
+    o[3] = (double)s.jobs_done;
+    o[4] = (double)s.max_active;
+    o[5] = s.busy_s;
+  });
+}
+
+int rs_server_destroy(rs_server *srv) {
+  if (!srv) return fail(RS_ERR_INVALID, "server is NULL");
+  return guarded([&] {
+    srv->srv->drain();
+    delete srv;
   });
 }
 

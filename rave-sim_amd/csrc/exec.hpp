@@ -1,0 +1,521 @@
+// Internal: launch records and the member-program compiler / executor shared
+// by the lockstep runner (engine.cpp) and the serving scheduler (serve.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+#include "rs_kernels.hpp"
+
+namespace rs {
+namespace exec {
+
+enum class LKind { Fiber, Xop, Kop, Detect, Copy, Fill, SaveU };
+
+struct Launch {
+  LKind kind;
+  int lgL = 0;
+  bool row = false;
+  Geom geom{};
+  size_t arg_off = 0;  // into FieldArgs / DetArgs arrays
+  int nf = 0;
+  // copy / fill / saveU
+  void *dst = nullptr;
+  const void *src = nullptr;
+  int64_t n = 0;
+  int member = 0;
+  int step = 0;
+};
+
+// pending state of one lockstep group
+struct PropAcc {
+  bool on = false;
+  double alpha = 0, beta = 0;
+  int64_t rs = 0, re = 0;
+  bool cut = false;
+};
+
+inline void prop_accumulate(PropAcc &a, const Op &op, double wl, int64_t N, double dx) {
+  const double L = (double)N * dx;
+  a.alpha += wl * op.dz / (2.0 * L * L);
+  // global phase exactly as big-wave evaluates it in FP64,
+  // np.exp(-1j * 2 * np.pi / wl * dz) (big-wave/propagation.py:126), then
+  // reduced to revolutions in extended precision
+  const double th = (-(2.0 * M_PI) / wl) * op.dz;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  long double red = fmodl((long double)th, two_pi) / two_pi;
+  double b = (double)(red - floorl(red));
+  a.beta += b;
+  a.beta -= std::floor(a.beta);
+  int64_t rs_, re_;
+  cutoff_range(N, op.fc, dx, &rs_, &re_);
+  if (rs_ != re_) {
+    if (!a.cut) {
+      a.rs = rs_;
+      a.re = re_;
+      a.cut = true;
+    } else {
+      a.rs = std::min(a.rs, rs_);
+      a.re = std::max(a.re, re_);
+    }
+  }
+  a.on = true;
+}
+
+inline KOp make_kop(const PropAcc &a, int64_t N, bool fold_scale) {
+  KOp k{};
+  k.on = 1;
+  k.alpha_rev = a.alpha;
+  k.beta_rev = a.beta;
+  k.rs = a.cut ? a.rs : 0;
+  k.re = a.cut ? a.re : 0;
+  k.scale[0] = fold_scale ? 1.0 / (double)N : 1.0;
+  k.scale[1] = 0.0;
+  return k;
+}
+
+inline XOp scalar_xop(double re, double im) {
+  XOp x{};
+  x.kind = XOP_SCALAR;
+  x.fa[0] = re;
+  x.fa[1] = im;
+  x.post[0] = 1.0;
+  x.post[1] = 0.0;
+  return x;
+}
+
+// x-space op composition: first `a`, then `b` (b is a MASK or SCALAR)
+inline XOp compose(const XOp &a, const XOp &b) {
+  if (a.kind == XOP_NONE) return b;
+  XOp r = a;
+  double f[2];
+  if (b.kind == XOP_SCALAR) {
+    f[0] = b.fa[0] * b.post[0] - b.fa[1] * b.post[1];
+    f[1] = b.fa[0] * b.post[1] + b.fa[1] * b.post[0];
+    const double p0 = r.post[0] * f[0] - r.post[1] * f[1];
+    const double p1 = r.post[0] * f[1] + r.post[1] * f[0];
+    r.post[0] = p0;
+    r.post[1] = p1;
+    return r;
+  }
+  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) && b.kind == XOP_MASK) {
+    XOp m = b;
+    // value of `a` at every sample: fa * post for a scalar, post for ones
+    const double s0 = a.kind == XOP_ONES ? a.post[0] : a.fa[0] * a.post[0] - a.fa[1] * a.post[1];
+    const double s1 = a.kind == XOP_ONES ? a.post[1] : a.fa[0] * a.post[1] + a.fa[1] * a.post[0];
+    const double p0 = m.post[0] * s0 - m.post[1] * s1;
+    const double p1 = m.post[0] * s1 + m.post[1] * s0;
+    m.post[0] = p0;
+    m.post[1] = p1;
+    if (a.kind == XOP_ONES || a.from_ones) m.from_ones = 1;
+    return m;
+  }
+  throw Error(6, "two x-space masks without a propagation in between");
+}
+
+template <typename S> struct Exec {
+  Context &ctx;
+  const RunConfig &rc;
+  Plan &plan;
+  std::vector<Member *> mem;          // this group
+  std::vector<void *> field;          // per member
+  std::vector<void *> snap;           // per member (phase stepping)
+  std::vector<FieldArgs> fargs;
+  std::vector<DetArgs> dargs;
+  std::vector<Launch> launches;
+  double *acc = nullptr;
+  const int64_t *bounds = nullptr;
+  int64_t npx = 0;
+
+  Exec(Context &c, const RunConfig &r, Plan &p) : ctx(c), rc(r), plan(p) {}
+
+  size_t push_args(const std::vector<FieldArgs> &v) {
+    const size_t off = fargs.size();
+    fargs.insert(fargs.end(), v.begin(), v.end());
+    return off;
+  }
+
+  void fiber(int lgL, bool row, const Geom &g, const std::vector<FieldArgs> &a) {
+    Launch l{LKind::Fiber};
+    l.lgL = lgL;
+    l.row = row;
+    l.geom = g;
+    l.nf = (int)a.size();
+    l.arg_off = push_args(a);
+    launches.push_back(l);
+  }
+
+  std::vector<FieldArgs> blank() const {
+    std::vector<FieldArgs> a(mem.size());
+    for (size_t i = 0; i < mem.size(); ++i) {
+      std::memset(&a[i], 0, sizeof(FieldArgs));
+      a[i].data = field[i];
+    }
+    return a;
+  }
+
+  // fused propagation from state X (transformed == false) or T
+  void emit_prop_fused(bool transformed, const std::vector<XOp> &px, const std::vector<PropAcc> &pa) {
+    const int K = (int)mem.size();
+    if (plan.ngroups == 1) {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) {
+        a[i].x = px[i];
+        a[i].do_fwd = 1;
+        a[i].do_inv = 1;
+        a[i].k = make_kop(pa[i], rc.N, true);
+      }
+      fiber(plan.lgR, true, plan.geom_row(), a);
+      return;
+    }
+    {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) {
+        a[i].x = px[i];
+        a[i].do_inv = transformed;
+        a[i].do_fwd = 1;
+      }
+      fiber(plan.lgA, false, plan.geom_top(), a);
+    }
+    if (plan.ngroups == 3) {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) a[i].do_fwd = 1;
+      fiber(plan.lgB, false, plan.geom_mid(), a);
+    }
+    {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) {
+        a[i].do_fwd = 1;
+        a[i].do_inv = 1;
+        a[i].k = make_kop(pa[i], rc.N, true);
+      }
+      fiber(plan.lgR, true, plan.geom_row(), a);
+    }
+    if (plan.ngroups == 3) {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) a[i].do_inv = 1;
+      fiber(plan.lgB, false, plan.geom_mid(), a);
+    }
+  }
+
+  // one unfused FFT direction (plain form)
+  void emit_fft_plain(bool inverse) {
+    const int K = (int)mem.size();
+    auto mk = [&](bool fwd, bool inv) {
+      auto a = blank();
+      for (int i = 0; i < K; ++i) { a[i].do_fwd = fwd; a[i].do_inv = inv; }
+      return a;
+    };
+    if (plan.ngroups == 1) {
+      fiber(plan.lgR, true, plan.geom_row(), mk(!inverse, inverse));
+      return;
+    }
+    Geom top = plan.geom_top();
+    top.top = 0;  // no x-space op inside an FFT pass of the plain form
+    if (!inverse) {
+      fiber(plan.lgA, false, top, mk(true, false));
+      if (plan.ngroups == 3) fiber(plan.lgB, false, plan.geom_mid(), mk(true, false));
+      fiber(plan.lgR, true, plan.geom_row(), mk(true, false));
+    } else {
+      fiber(plan.lgR, true, plan.geom_row(), mk(false, true));
+      if (plan.ngroups == 3) fiber(plan.lgB, false, plan.geom_mid(), mk(false, true));
+      fiber(plan.lgA, false, top, mk(false, true));
+    }
+  }
+
+  void emit_xop(const std::vector<XOp> &px) {
+    auto a = blank();
+    for (size_t i = 0; i < mem.size(); ++i) a[i].x = px[i];
+    Launch l{LKind::Xop};
+    l.nf = (int)a.size();
+    l.arg_off = push_args(a);
+    launches.push_back(l);
+  }
+
+  void emit_kop(const std::vector<KOp> &ks) {
+    auto a = blank();
+    for (size_t i = 0; i < mem.size(); ++i) a[i].k = ks[i];
+    Launch l{LKind::Kop};
+    l.geom = plan.geom_row();
+    l.nf = (int)a.size();
+    l.arg_off = push_args(a);
+    launches.push_back(l);
+  }
+
+  // flush a transformed field back to natural order (fused form)
+  void emit_to_natural(const std::vector<XOp> &px) {
+    auto a = blank();
+    for (size_t i = 0; i < mem.size(); ++i) {
+      a[i].x = px[i];
+      a[i].do_inv = 1;
+    }
+    fiber(plan.lgA, false, plan.geom_top(), a);
+  }
+
+  void compile() {
+    const int K = (int)mem.size();
+    const size_t nops = mem[0]->ops.size();
+    for (int i = 1; i < K; ++i)
+      if (mem[i]->ops.size() != nops) throw Error(1, "lockstep members differ in length");
+    std::vector<XOp> px(K);
+    for (auto &x : px) std::memset(&x, 0, sizeof(XOp));
+    std::vector<PropAcc> pa(K);
+    bool transformed = false;
+    const bool plain = rc.mode == 1;
+    bool snap_transformed = false;
+
+    auto clear_px = [&]() {
+      for (auto &x : px) std::memset(&x, 0, sizeof(XOp));
+    };
+    auto flush_natural = [&]() {
+      bool any = false;
+      for (auto &x : px) any |= x.kind != XOP_NONE;
+      if (transformed) {
+        emit_to_natural(px);
+        transformed = false;
+      } else if (any) {
+        emit_xop(px);
+      }
+      clear_px();
+    };
+
+    for (size_t o = 0; o < nops; ++o) {
+      const OpKind kind = mem[0]->ops[o].kind;
+      for (int i = 1; i < K; ++i)
+        if (mem[i]->ops[o].kind != kind) throw Error(1, "lockstep members differ in op kinds");
+      switch (kind) {
+        case OpKind::Source:
+          for (int i = 0; i < K; ++i) px[i] = mem[i]->ops[o].x;
+          transformed = false;
+          break;
+        case OpKind::Upload: {
+          for (int i = 0; i < K; ++i) {
+            Launch l{LKind::Copy};
+            l.dst = field[i];
+            l.src = mem[i]->upload;
+            l.n = rc.N;
+            l.member = -1;  // host to device
+            launches.push_back(l);
+          }
+          clear_px();
+          transformed = false;
+          break;
+        }
+        case OpKind::Mask:
+          for (int i = 0; i < K; ++i) px[i] = compose(px[i], mem[i]->ops[o].x);
+          break;
+        case OpKind::Prop: {
+          for (int i = 0; i < K; ++i) prop_accumulate(pa[i], mem[i]->ops[o], mem[i]->wl, rc.N, rc.dx);
+          const bool next_prop = (o + 1 < nops) && mem[0]->ops[o + 1].kind == OpKind::Prop;
+          if (next_prop && !plain) break;  // merge consecutive propagations
+          if (plain) {
+            bool any = false;
+            for (auto &x : px) any |= x.kind != XOP_NONE;
+            if (any) emit_xop(px);
+            emit_fft_plain(false);
+            std::vector<KOp> ks(K);
+            for (int i = 0; i < K; ++i) ks[i] = make_kop(pa[i], rc.N, false);
+            emit_kop(ks);
+            emit_fft_plain(true);
+            std::vector<XOp> sc(K, scalar_xop(1.0 / (double)rc.N, 0.0));
+            emit_xop(sc);
+          } else {
+            emit_prop_fused(transformed, px, pa);
+            transformed = plan.ngroups > 1;
+          }
+          clear_px();
+          for (auto &a : pa) a = PropAcc{};
+          break;
+        }
+        case OpKind::Detect: {
+          flush_natural();
+          const size_t off = dargs.size();
+          for (int i = 0; i < K; ++i) {
+            DetArgs d{};
+            d.data = field[i];
+            d.acc = acc + (size_t)(mem[i]->out_base + mem[i]->ops[o].row) * npx;
+            d.weight = mem[i]->weight;
+            d.accumulate = 1;
+            dargs.push_back(d);
+          }
+          Launch l{LKind::Detect};
+          l.nf = K;
+          l.arg_off = off;
+          launches.push_back(l);
+          break;
+        }
+        case OpKind::Flush:
+          flush_natural();
+          break;
+        case OpKind::Snapshot: {
+          bool any = false;
+          for (auto &x : px) any |= x.kind != XOP_NONE;
+          if (any) flush_natural();
+          if (!snap.empty()) {
+            for (int i = 0; i < K; ++i) {
+              Launch l{LKind::Copy};
+              l.dst = snap[i];
+              l.src = field[i];
+              l.n = rc.N;
+              launches.push_back(l);
+            }
+          }
+          snap_transformed = transformed;
+          break;
+        }
+        case OpKind::Restore:
+          for (int i = 0; i < K; ++i) {
+            Launch l{LKind::Copy};
+            l.dst = field[i];
+            l.src = snap[i];
+            l.n = rc.N;
+            launches.push_back(l);
+          }
+          clear_px();
+          transformed = snap_transformed;
+          break;
+        case OpKind::SaveU: {
+          flush_natural();
+          for (int i = 0; i < K; ++i) {
+            Launch l{LKind::SaveU};
+            l.src = field[i];
+            l.n = rc.N;
+            l.member = i;
+            l.step = mem[i]->ops[o].row;
+            launches.push_back(l);
+          }
+          break;
+        }
+      }
+    }
+  }
+
+  void execute() {
+    hipStream_t st = ctx.stream;
+    DevBuf dfa(std::max<size_t>(fargs.size(), 1) * sizeof(FieldArgs));
+    DevBuf dda(std::max<size_t>(dargs.size(), 1) * sizeof(DetArgs));
+    if (!fargs.empty())
+      hip_check(hipMemcpyAsync(dfa.p, fargs.data(), fargs.size() * sizeof(FieldArgs), hipMemcpyHostToDevice, st),
+                "args upload");
+    if (!dargs.empty())
+      hip_check(hipMemcpyAsync(dda.p, dargs.data(), dargs.size() * sizeof(DetArgs), hipMemcpyHostToDevice, st),
+                "det args upload");
+    const FieldArgs *fa = (const FieldArgs *)dfa.p;
+    const DetArgs *da = (const DetArgs *)dda.p;
+    const size_t esz = 2 * sizeof(S);
+
+    // graph capture covers the run of launches between host interactions
+    bool capturing = false;
+    hipGraph_t graph = nullptr;
+    auto begin_capture = [&]() {
+      if (rc.use_graph && !capturing) {
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+        capturing = true;
+      }
+    };
+    auto end_capture = [&]() {
+      if (capturing) {
+        hip_check(hipStreamEndCapture(st, &graph), "end capture");
+        capturing = false;
+        hipGraphExec_t ex = nullptr;
+        hip_check(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0), "graph instantiate");
+        hip_check(hipGraphLaunch(ex, st), "graph launch");
+        hip_check(hipStreamSynchronize(st), "graph sync");
+        (void)hipGraphExecDestroy(ex);
+        (void)hipGraphDestroy(graph);
+        graph = nullptr;
+      }
+    };
+
+    // optional per-launch event timing (not combined with graph capture)
+    const bool timed = ctx.timing && !rc.use_graph;
+    std::vector<hipEvent_t> ev;
+    std::vector<std::pair<int, double>> ev_meta;  // class, bytes
+    auto t_begin = [&](int cls, double bytes) {
+      if (!timed) return;
+      hipEvent_t a, b;
+      hip_check(hipEventCreate(&a), "event");
+      hip_check(hipEventCreate(&b), "event");
+      ev.push_back(a);
+      ev.push_back(b);
+      ev_meta.emplace_back(cls, bytes);
+      hip_check(hipEventRecord(a, st), "event record");
+    };
+    auto t_end = [&]() {
+      if (timed) hip_check(hipEventRecord(ev.back(), st), "event record");
+    };
+
+    for (const Launch &l : launches) {
+      switch (l.kind) {
+        case LKind::Fiber: {
+          begin_capture();
+          const int cls = l.row ? PC_ROW : (l.geom.top ? PC_TOP : PC_MID);
+          t_begin(cls, 2.0 * (double)rc.N * (double)esz * l.nf);
+          hip_check(launch_fiber_pass<S>(l.lgL, l.row, fa + l.arg_off, l.nf, l.geom, st), "fiber pass");
+          t_end();
+          break;
+        }
+        case LKind::Xop:
+          begin_capture();
+          t_begin(PC_OTHER, 2.0 * (double)rc.N * (double)esz * l.nf);
+          hip_check(launch_xop<S>(fa + l.arg_off, l.nf, rc.N, st), "xop");
+          t_end();
+          break;
+        case LKind::Kop:
+          begin_capture();
+          t_begin(PC_OTHER, 2.0 * (double)rc.N * (double)esz * l.nf);
+          hip_check(launch_kop<S>(fa + l.arg_off, l.nf, l.geom, st), "kop");
+          t_end();
+          break;
+        case LKind::Detect:
+          begin_capture();
+          hip_check(launch_detect<S>(da + l.arg_off, l.nf, bounds, npx, st), "detect");
+          break;
+        case LKind::Copy:
+          if (l.member == -1) {
+            end_capture();
+            hip_check(hipMemcpyAsync(l.dst, l.src, l.n * esz, hipMemcpyHostToDevice, st), "upload");
+          } else {
+            begin_capture();
+            hip_check(launch_copy<S>(l.dst, l.src, l.n, st), "copy");
+          }
+          break;
+        case LKind::Fill:
+          begin_capture();
+          hip_check(launch_fill_f64((double *)l.dst, l.n, 0.0, st), "fill");
+          break;
+        case LKind::SaveU: {
+          end_capture();
+          Member *m = mem[l.member];
+          if (m->saved_u) {
+            std::vector<char> host(l.n * esz);
+            hip_check(hipMemcpyAsync(host.data(), l.src, l.n * esz, hipMemcpyDeviceToHost, st), "download u");
+            hip_check(hipStreamSynchronize(st), "sync");
+            if ((int)m->saved_u->size() <= l.step) m->saved_u->resize(l.step + 1);
+            (*m->saved_u)[l.step] = std::move(host);
+          }
+          break;
+        }
+      }
+    }
+    end_capture();
+    hip_check(hipStreamSynchronize(st), "run sync");
+    for (size_t i = 0; i < ev_meta.size(); ++i) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "event elapsed");
+      PassStat &p = ctx.stats[ev_meta[i].first];
+      p.launches += 1;
+      p.ms += ms;
+      p.bytes += ev_meta[i].second;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+
+}  // namespace exec
+}  // namespace rs

@@ -1,0 +1,92 @@
+// Serving: continuous batching of simulation requests on one device.
+//
+// A request is a phase-stepping sweep (or a single exposure) of one setup at
+// one or more energies.  Each (request, energy) pair is a *job*: one member
+// program (simulation.cpp build_member) with its own field, snapshot and
+// detector rows in device memory.  Jobs are admitted in arrival order while
+// their memory fits the device budget (hipMemGetInfo at start minus a
+// reserve), so fields stay resident from admission to completion.
+//
+// The scheduler thread advances every admitted job by one pass per tick and
+// merges, per tick, the passes of all jobs that need the same kernel (same
+// plan, pass kind and fiber length) into one launch (blockIdx.y = job).  A job
+// admitted while others are mid-way simply joins the next tick's launches.
+// Every job writes its own detector rows; a request's result is the weighted
+// sum over its jobs taken on the host in energy order, so it does not depend
+// on which other requests shared the launches.
+#pragma once
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+#include "simulation.hpp"
+
+namespace rs {
+
+struct ServeRequest {
+  std::vector<SimConfig> members;  // one per energy (same N, precision, detector, phase steps)
+  std::vector<double> weights;
+};
+
+struct ServeResult {
+  int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  std::vector<double> rows;  // n_steps x n_px
+  int64_t n_steps = 0, n_px = 0;
+  double submit_t = 0, start_t = 0, done_t = 0;
+  std::string error;
+};
+
+struct ServeStats {
+  int64_t ticks = 0, launches = 0, merged_jobs = 0;  // merged_jobs: sum over launches of jobs per launch
+  int64_t jobs_done = 0, max_active = 0;
+  double busy_s = 0;
+};
+
+class Server {
+ public:
+  Server(Context &ctx, double reserve_bytes, int max_jobs);
+  ~Server();
+  int64_t submit(ServeRequest req);
+  ServeResult poll(int64_t id);
+  bool wait(int64_t id, double timeout_s);
+  void drain();
+  ServeStats stats();
+
+ private:
+  struct Job;
+  struct Req;
+  void loop();
+  void admit();
+  bool tick();
+  void reap();
+  template <typename S> void compile_job(Job &j);
+
+  Context &ctx_;
+  int max_jobs_;
+  struct Pool;
+  std::unique_ptr<Pool> pool_;
+  size_t budget_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<int64_t> pending_;
+  std::map<int64_t, std::shared_ptr<Req>> reqs_;
+  std::vector<std::unique_ptr<Job>> active_;
+  int64_t next_id_ = 1;
+  bool stop_ = false;
+  ServeStats stats_;
+  std::thread th_;
+  // args staging ring
+  struct Slot;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  size_t slot_i_ = 0;
+};
+
+}  // namespace rs

@@ -33,6 +33,8 @@ EXPORTS = (
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
     "rs_apply_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
+    "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
+    "rs_server_destroy",
 )
 
 
@@ -80,6 +82,15 @@ class rs_batch_desc(C.Structure):
                 ("use_graph", C.c_int32), ("reserved", C.c_int32)]
 
 
+class rs_request(C.Structure):
+    _fields_ = [("grid", rs_grid), ("det", rs_detector), ("z_detector", C.c_double),
+                ("dtype", C.c_int32), ("nr_gratings", C.c_int32), ("nr_energies", C.c_int32),
+                ("source_kind", C.c_int32), ("energies_ev", C.POINTER(C.c_double)),
+                ("weights", C.POINTER(C.c_double)), ("gratings", C.POINTER(rs_grating)),
+                ("cutoff_angles", C.POINTER(C.c_double)), ("source_x", C.c_double),
+                ("source_z", C.c_double)]
+
+
 class rs_pass_stat(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
@@ -115,6 +126,13 @@ def _load() -> C.CDLL:
         "rs_set_timing": (C.c_int, [P, C.c_int]),
         "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "rs_set_split": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.c_int]),
+        "rs_server_create": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(P)]),
+        "rs_server_submit": (C.c_int, [P, C.POINTER(rs_request), C.POINTER(C.c_int64)]),
+        "rs_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int64,
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "rs_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(C.c_int32)]),
+        "rs_server_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "rs_server_destroy": (C.c_int, [P]),
         "rs_get_pass_stats": (C.c_int, [P, C.POINTER(rs_pass_stat), C.c_int]),
     }
     for name, (res, args) in sig.items():

@@ -1,0 +1,92 @@
+"""Serving form: continuous batching of requests of different sizes.
+
+Checks that a request's detector rows match the oracle, and that they are
+bit-for-bit the same whether the request is served alone or while other
+requests (other N, other energies) share its launches."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import bigwave as bw
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_sweep(req):
+    """Weighted incoherent sum over the request's energies, each a phase-stepped
+    oracle run (big-wave/wavesim.py:84-298) of the same setup."""
+    total = None
+    for e, w, grs in zip(req.energies_ev, req.weights or [1.0] * len(req.energies_ev), req.gratings):
+        wl = bw.convert_energy_wavelength(e)
+        p = bw.SimParams(N=req.N, dx=req.dx, z_detector=req.z_detector, detector_size=req.det_size,
+                         detector_pixel_size_x=req.det_px, detector_pixel_size_y=req.det_py, wl=wl,
+                         chunk_size=req.N)
+        els = [bw.Grating(pitch=g.pitch, dc=tuple(g.dc), z_start=g.z_start, thickness=g.thickness,
+                          nr_steps=g.nr_steps, x_positions=np.array(g.x_positions, dtype=float),
+                          substrate_thickness=g.substrate_thickness, db_a=g.db_a, db_b=g.db_b,
+                          db_substrate=g.db_substrate) for g in grs]
+        rows = np.array(bw.run_simulation(p, bw.VectorSource(np.ones(req.N, complex), 0.0), els,
+                                          list(req.cutoff_angles)))
+        total = w * rows if total is None else total + w * rows
+    return total
+
+
+@pytest.fixture(scope="module")
+def server():
+    import ravesim
+    from ravesim.serve import Server
+    ctx = ravesim.Context(0)
+    s = Server(ctx)
+    yield s
+    s.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 1e-4)])
+def test_served_sweep_matches_oracle(server, dtype, tol):
+    from ravesim.serve import phase_sweep
+    req = phase_sweep(2 ** 16, [40000.0, 52000.0], n_steps=4, slices=4, dtype=dtype, weights=[0.3, 0.7])
+    got = server.result(server.submit(req)).rows
+    ref = oracle_sweep(req)
+    assert got.shape == ref.shape == (4, 65)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
+def test_alone_and_in_the_mix_are_identical(server):
+    from ravesim.serve import phase_sweep
+    a = phase_sweep(2 ** 18, [46000.0, 48000.0, 50000.0], n_steps=8, slices=6)
+    alone = server.result(server.submit(a)).rows
+    others = [phase_sweep(2 ** 16, [30000.0 + 1000 * i], n_steps=8, slices=6) for i in range(3)]
+    others += [phase_sweep(2 ** 20, [60000.0], n_steps=8, slices=3)]
+    ids = [server.submit(others[0]), server.submit(others[3])]
+    time.sleep(0.01)
+    ida = server.submit(a)
+    ids += [server.submit(o) for o in others[1:3]]
+    mixed = server.result(ida).rows
+    for i in ids:
+        server.result(i)
+    st = server.stats()
+    assert st["jobs_per_launch"] > 1.0  # launches were shared
+    assert np.array_equal(alone, mixed)
+
+
+def test_admission_with_small_budget_completes():
+    """A budget that holds one job at a time still serves every request."""
+    import ctypes as C
+    import ravesim
+    from ravesim.serve import Server, phase_sweep
+    ctx = ravesim.Context(0)
+    free = C.c_size_t()
+    total = C.c_size_t()
+    lib = C.CDLL("libamdhip64.so")
+    lib.hipMemGetInfo(C.byref(free), C.byref(total))
+    one_job = 2 * 2 ** 18 * 8 + 8 * 100 * 8
+    s = Server(ctx, reserve_bytes=float(free.value - int(1.5 * one_job)))
+    reqs = [phase_sweep(2 ** 18, [46000.0 + 500 * i], n_steps=2, slices=2) for i in range(3)]
+    ids = [s.submit(r) for r in reqs]
+    outs = [s.result(i, timeout_s=120).rows for i in ids]
+    assert all(np.isfinite(o).all() and o.shape == (2, 100) for o in outs)
+    assert s.stats()["max_resident_jobs"] == 1
+    s.close()
+    ctx.close()
