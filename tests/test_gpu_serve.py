@@ -86,7 +86,8 @@ def test_admission_with_small_budget_completes():
     reqs = [phase_sweep(2 ** 18, [46000.0 + 500 * i], n_steps=2, slices=2) for i in range(3)]
     ids = [s.submit(r) for r in reqs]
     outs = [s.result(i, timeout_s=120).rows for i in ids]
-    assert all(np.isfinite(o).all() and o.shape == (2, 100) for o in outs)
+    npx = int(reqs[0].det_size // reqs[0].det_px)
+    assert all(np.isfinite(o).all() and o.shape == (2, npx) for o in outs)
     assert s.stats()["max_resident_jobs"] == 1
     s.close()
     ctx.close()
