@@ -43,6 +43,89 @@ def cfg2_members(ravesim, N, slices=64, plain=False):
     return [m], 0.4 + h, 3 * slices + 2
 
 
+def cfg3_members(ravesim, N, n_energies=64, slices=32):
+    """Polychromatic batch: 64 energies 20-70 keV on the single-grating chain
+    (Si phase grating p 4.2 um, h 30 um, 32 slices, 0.1 m hop), weights of a
+    70 kVp tungsten spectrum (Kramers (70-E)/E times 3 mm Al, mu ~ E^-3,
+    0.5/cm at 50 keV), delta ~ E^-2 and beta ~ E^-4 from the 25 keV values."""
+    import numpy as np
+    e = np.linspace(20.0, 70.0, n_energies)
+    w = (70.0 - e) / e * np.exp(-0.5 * (e / 50.0) ** -3 * 0.3)
+    w = np.clip(w, 0, None)
+    w = w / w.sum()
+    h = 30e-6
+    ms = []
+    for ek, wk in zip(e, w):
+        sc = 25.0 / ek
+        g = ravesim.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=h, nr_steps=slices,
+                            db_a=complex(7.7e-7 * sc ** 2, 3.5e-10 * sc ** 4))
+        ms.append(ravesim.core.BatchMember(wl=ravesim.energy_to_wavelength(ek * 1000.0), gratings=[g],
+                                           cutoff_angles=[math.pi / 2] * 2, weight=float(wk), plane_wave=True))
+    return ms, h + 0.1, slices + 1
+
+
+def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
+    """Batched fused complex64 form vs the plain complex128 form run one
+    simulation at a time, on the polychromatic batch (sims/hour each)."""
+    import numpy as np
+    from ravesim.core import run_batch
+    N = 2 ** lgN
+    ms, zdet, steps = cfg3_members(ravesim, N)
+    kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet)
+    tuned = ctx.autotune(N, np.complex64)
+    run_batch(ctx, N, 2e-9, ms[:2], dtype=np.complex64, **kw)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fused = run_batch(ctx, N, 2e-9, ms, dtype=np.complex64, **kw)
+    tf = (time.perf_counter() - t0) / reps
+    run_batch(ctx, N, 2e-9, ms[:1], dtype=np.complex128, plain=True, max_resident=1, **kw)
+    t0 = time.perf_counter()
+    plain = run_batch(ctx, N, 2e-9, ms, dtype=np.complex128, plain=True, max_resident=1, **kw)
+    tp = time.perf_counter() - t0
+    err = float(np.linalg.norm(fused - plain) / np.linalg.norm(plain))
+    return {"workload": f"64 energies x N=2^{lgN}, {steps} steps each, weighted detector sum",
+            "fused_c64_batched_sims_per_hour": round(len(ms) / tf * 3600, 1),
+            "plain_c128_one_at_a_time_sims_per_hour": round(len(ms) / tp * 3600, 1),
+            "speedup": round(tp / tf, 2), "rel_l2_fused_vs_plain_c128": err, "split": tuned}
+
+
+def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_energies=4):
+    """Poisson arrivals of Talbot-Lau phase-stepping requests (8 G2 positions,
+    3 x 64 slices) at a fixed rate; N log-uniform over `sizes`, 1..max_energies
+    energies uniform on 20-70 keV.  Returns latency percentiles and throughput."""
+    import numpy as np
+    from ravesim.serve import Server, phase_sweep
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / qps)
+        if t > duration_s:
+            break
+        n = 2 ** int(rng.choice(sizes))
+        ne = int(rng.integers(1, max_energies + 1))
+        arrivals.append((t, phase_sweep(n, list(rng.uniform(20e3, 70e3, ne)), n_steps=8, slices=64)))
+    srv = Server(ctx)
+    t0 = time.perf_counter()
+    ids = []
+    for at, req in arrivals:
+        dt = at - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append((srv.submit(req), len(req.energies_ev)))
+    res = [srv.result(i, timeout_s=600) for i, _ in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r.latency_s for r in res])
+    sims = sum(ne for _, ne in ids)
+    return {"qps": qps, "requests": len(res), "sims": sims, "duration_s": round(wall, 2),
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "sims_per_hour": round(sims / wall * 3600, 1),
+            "mix": f"N in 2^{list(sizes)}, 1-{max_energies} energies, 8 phase steps, 650 steps per energy",
+            "jobs_per_launch": round(st["jobs_per_launch"], 2), "max_resident_jobs": st["max_resident_jobs"]}
+
+
 def cpu_baseline(N, budget_s=25.0):
     """Oracle (complex128 numpy/scipy restatement, 1 scipy worker) on whole
     grating slices of G0 at the benchmark N; scaled to sims/hour."""
@@ -81,6 +164,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
     ap.add_argument("--split", default="", help="fixed digit-group split 'lgR,lgA' instead of tuning")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--serve-qps", type=float, default=4.0, help="serving run arrival rate (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -139,6 +225,11 @@ def main():
                 "passes": {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
+    batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx)
+    serving = None
+    if args.serve_qps > 0 and not args.plain:
+        serving = serving_run(ravesim, ctx, args.serve_qps, args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N)
@@ -158,6 +249,8 @@ def main():
         "hbm_frac_whole_sim": round(gss * 1e9 * (32 if args.dtype == "c8" else 64) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_vs_plain": batch,
+        "serving": serving,
         "detector_checksum": float(np.sum(out)),
     }
     print(json.dumps(line), flush=True)
