@@ -1,0 +1,480 @@
+// Device building blocks of the gfx950 kernels: complex arithmetic, in-register
+// DFTs, radix-16 Stockham stages through LDS, twiddle generation, and the
+// elementwise operations of the split step (x-space grating / source, spectrum
+// propagator).
+//
+// Reference counterparts: fast-wave/fwcuda/kernels.cu (propagate_analytically
+// :6-28, propagate_convolve_step :40-65, apply_grating_factors :67-89) and the
+// cuFFT calls of fast-wave/src/fft.cpp:28-44; numerics follow big-wave
+// (propagation.py, optical_element.py).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rs_device.hpp"
+
+namespace rs {
+
+template <typename S> struct C2T;
+template <> struct C2T<float> { using t = float2; };
+template <> struct C2T<double> { using t = double2; };
+
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  T2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cmulc(T2 a, T2 b) {  // a * conj(b)
+  T2 r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.y * b.x - a.x * b.y;
+  return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) {
+  T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) {
+  T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename T2> __device__ __forceinline__ T2 mul_mi(T2 a) {
+  T2 r;
+  if (INV) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+  return r;
+}
+
+// element at a byte offset from a workgroup-uniform pointer: keeps the
+// per-lane part of the address in 32 bits (base in SGPRs + 32-bit offset)
+template <typename T2> __device__ __forceinline__ T2 &at(T2 *p, uint32_t off_bytes) {
+  return *(T2 *)((char *)p + off_bytes);
+}
+
+// ---------------------------------------------------------------------------
+// in-register DFTs, natural order in and out, W = exp(-2 pi i / R) (forward)
+// ---------------------------------------------------------------------------
+
+template <bool INV, typename T2> __device__ __forceinline__ void dft2(T2 &a, T2 &b) {
+  T2 t = a; a = cadd(t, b); b = csub(t, b);
+}
+
+template <bool INV, typename T2> __device__ __forceinline__ void dft4(T2 &x0, T2 &x1, T2 &x2, T2 &x3) {
+  T2 t0 = cadd(x0, x2), t1 = csub(x0, x2);
+  T2 t2 = cadd(x1, x3), t3 = mul_mi<INV>(csub(x1, x3));
+  x0 = cadd(t0, t2); x2 = csub(t0, t2);
+  x1 = cadd(t1, t3); x3 = csub(t1, t3);
+}
+
+#define RS_SQRT1_2 0.70710678118654752440
+#define RS_COS_PI8 0.92387953251128675613
+#define RS_SIN_PI8 0.38268343236508977173
+
+// rotate by W8^j for j=1,2,3 (forward sign; conjugate for inverse)
+template <bool INV, typename T2> __device__ __forceinline__ T2 w8(T2 a, int j) {
+  using S = decltype(a.x);
+  const S h = (S)RS_SQRT1_2;
+  T2 r;
+  if (j == 1) {  // (1 - i)/sqrt2
+    if (!INV) { r.x = (a.x + a.y) * h; r.y = (a.y - a.x) * h; }
+    else { r.x = (a.x - a.y) * h; r.y = (a.y + a.x) * h; }
+  } else if (j == 2) {
+    r = mul_mi<INV>(a);
+  } else {  // W8^3 = (-1 - i)/sqrt2
+    if (!INV) { r.x = (a.y - a.x) * h; r.y = -(a.x + a.y) * h; }
+    else { r.x = -(a.x + a.y) * h; r.y = (a.x - a.y) * h; }
+  }
+  return r;
+}
+
+template <bool INV, typename T2> __device__ __forceinline__ void dft8(T2 *x) {
+  T2 a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6];
+  T2 b0 = x[1], b1 = x[3], b2 = x[5], b3 = x[7];
+  dft4<INV>(a0, a1, a2, a3);
+  dft4<INV>(b0, b1, b2, b3);
+  b1 = w8<INV>(b1, 1); b2 = w8<INV>(b2, 2); b3 = w8<INV>(b3, 3);
+  x[0] = cadd(a0, b0); x[4] = csub(a0, b0);
+  x[1] = cadd(a1, b1); x[5] = csub(a1, b1);
+  x[2] = cadd(a2, b2); x[6] = csub(a2, b2);
+  x[3] = cadd(a3, b3); x[7] = csub(a3, b3);
+}
+
+// W16^j for j in 0..9 as (cos, sin) of -2 pi j/16
+template <bool INV, typename T2> __device__ __forceinline__ T2 w16(T2 a, int j) {
+  using S = decltype(a.x);
+  const S c1 = (S)RS_COS_PI8, s1 = (S)RS_SIN_PI8, h = (S)RS_SQRT1_2;
+  S c, s;  // forward: multiply by c - i s
+  switch (j) {
+    case 0: return a;
+    case 1: c = c1; s = s1; break;
+    case 2: c = h; s = h; break;
+    case 3: c = s1; s = c1; break;
+    case 4: return mul_mi<INV>(a);
+    case 6: c = -h; s = h; break;
+    case 9: c = -c1; s = -s1; break;
+    default: c = 1; s = 0; break;
+  }
+  if (INV) s = -s;
+  T2 r;
+  r.x = a.x * c + a.y * s;
+  r.y = a.y * c - a.x * s;
+  return r;
+}
+
+template <bool INV, typename T2> __device__ __forceinline__ void dft16(T2 *x) {
+  // X[q + 4s] = sum_p W4^{ps} W16^{pq} sum_m x[p + 4m] W4^{mq}
+  T2 g[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    g[p][0] = x[p]; g[p][1] = x[p + 4]; g[p][2] = x[p + 8]; g[p][3] = x[p + 12];
+    dft4<INV>(g[p][0], g[p][1], g[p][2], g[p][3]);
+  }
+#pragma unroll
+  for (int p = 1; p < 4; ++p)
+#pragma unroll
+    for (int q = 1; q < 4; ++q) g[p][q] = w16<INV>(g[p][q], p * q);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    T2 h0 = g[0][q], h1 = g[1][q], h2 = g[2][q], h3 = g[3][q];
+    dft4<INV>(h0, h1, h2, h3);
+    x[q] = h0; x[q + 4] = h1; x[q + 8] = h2; x[q + 12] = h3;
+  }
+}
+
+template <int R, bool INV, typename T2> __device__ __forceinline__ void dftR(T2 *x) {
+  if constexpr (R == 2) dft2<INV>(x[0], x[1]);
+  else if constexpr (R == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
+  else if constexpr (R == 8) dft8<INV>(x);
+  else dft16<INV>(x);
+}
+
+// ---------------------------------------------------------------------------
+// Stockham stages over one fiber of length L held by L/16 threads.
+// Thread q holds positions q + m*(L/16), m < 16, in v[m] before the first and
+// after the last stage; intermediate stages go through the fiber's LDS slab.
+// ---------------------------------------------------------------------------
+
+// LDS position of fiber element i: one pad word per 16 (bank spread)
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+// lds_pad(i + d) - lds_pad(i) for the strides used below: d = r * D with D a
+// power of two and i < D or i a multiple of 16 * ceil(D / 16) plus less than
+// D, so the pad of the sum splits and the offset is a compile-time constant
+// (ds_read / ds_write immediate offsets)
+template <int D> constexpr int lds_step(int r) { return r * D + ((r * D) >> 4); }
+
+// w[r] = W^(r e) for r < R from the table W[j] = exp(-2 pi i j / len),
+// index mask len-1: reads e, 2e, 4e, 8e and forms the other powers as one
+// product of two read values (error <= ~2 ulp, no long recurrences).
+template <typename T2, int R>
+__device__ __forceinline__ void twiddle_powers(const T2 *__restrict__ tab, int e, int mask, T2 *w) {
+  w[0].x = 1;
+  w[0].y = 0;
+  if constexpr (R >= 2) w[1] = tab[e & mask];
+  if constexpr (R >= 4) {
+    w[2] = tab[(2 * e) & mask];
+    w[3] = cmul(w[2], w[1]);
+  }
+  if constexpr (R >= 8) {
+    w[4] = tab[(4 * e) & mask];
+    w[5] = cmul(w[4], w[1]);
+    w[6] = cmul(w[4], w[2]);
+    w[7] = cmul(w[4], w[3]);
+  }
+  if constexpr (R >= 16) {
+    w[8] = tab[(8 * e) & mask];
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[8], w[r - 8]);
+  }
+}
+
+// split-table twiddle W_M^e = hi[e >> sh] * lo[e & (2^sh - 1)], M = 2^lgM
+template <typename T2>
+__device__ __forceinline__ T2 split_twiddle(const T2 *__restrict__ hi, const T2 *__restrict__ lo, int sh,
+                                            int64_t e) {
+  return cmul(hi[e >> sh], lo[e & ((int64_t(1) << sh) - 1)]);
+}
+
+template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV>
+__device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
+                                               const T2 *__restrict__ twf) {
+  constexpr int TPF = L / 16;
+  constexpr int NB = 16 / R;
+  constexpr int LR = L / R;
+  T2 x[NB][R];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = q + i * TPF;
+    if constexpr (FIRST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[i][r] = v[i + r * NB];
+    } else {
+      // R = 16 here (the remainder radix only starts a transform): b < L/R
+      const T2 *src = fib + lds_pad(b);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[i][r] = src[lds_step<LR>(r)];
+      // W_{NS R}^{r k} for r < R from a few table reads (k, 2k, 4k, 8k);
+      // the remaining powers are products of at most two of them
+      const int e = (b % NS) * (L / (NS * R));
+      T2 w[R];
+      twiddle_powers<T2, R>(twf, e, L - 1, w);
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[i][r] = INV ? cmulc(x[i][r], w[r]) : cmul(x[i][r], w[r]);
+    }
+    dftR<R, INV>(x[i]);
+  }
+  if constexpr (LAST) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i + r * NB] = x[i][r];
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = q + i * TPF;
+      const int k = b % NS;
+      const int o = (b / NS) * NS * R + k;
+      T2 *dst = fib + lds_pad(o);
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[lds_step<NS>(r)] = x[i][r];
+    }
+    __syncthreads();
+  }
+}
+
+// radix of the stage that starts at sub-length NS: the remainder radix
+// (L = R0 * 16^k) goes first, where it needs no twiddles
+template <int L, int NS> constexpr int stage_radix() {
+  if (NS != 1) return 16;
+  int r = 1, l = L;
+  while (l > 16) l /= 16;
+  r = l;  // L = r * 16^k with r in {2, 4, 8, 16}
+  return r;
+}
+
+template <typename T2, int L, int NS, bool INV>
+__device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf) {
+  constexpr int R = stage_radix<L, NS>();
+  constexpr bool FIRST = NS == 1;
+  constexpr bool LAST = NS * R == L;
+  stockham_stage<T2, L, R, NS, FIRST, LAST, INV>(v, fib, q, twf);
+  if constexpr (!LAST) stockham<T2, L, NS * R, INV>(v, fib, q, twf);
+}
+
+// ---------------------------------------------------------------------------
+// x-space operations
+// ---------------------------------------------------------------------------
+
+// Any x-space op on natural index n.  The grating coordinate and mask follow
+// big-wave bit for bit: x = linspace(x0, x0 + len*dx, len, endpoint=False)
+// per chunk, mask = mod(x/pitch + dc/2, 1) < dc, all in FP64 without
+// contraction (big-wave/optical_element.py:341-357).
+template <typename T2>
+__device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 v) {
+#pragma clang fp contract(off)
+  using S = decltype(v.x);
+  double re = 0.0, im = 0.0;
+  bool mul = true;
+  switch (op.kind) {
+    case XOP_MASK: {
+      double x;
+      if (op.single_chunk) {
+        x = (double)n * op.xstep + op.x0;
+      } else {
+        const int64_t c = n / op.chunk;
+        const int64_t k = n - c * op.chunk;
+        int64_t len = N - c * op.chunk;
+        if (len > op.chunk) len = op.chunk;
+        const int64_t start = c * op.chunk - N / 2;
+        const double x_start = (double)start * op.dx + op.xpos;
+        const double x_stop = x_start + (double)len * op.dx;
+        const double step = (x_stop - x_start) / (double)len;
+        x = (double)k * step + x_start;
+      }
+      // t = x/pitch + dc/2 as big-wave rounds it.  x * (1/pitch) differs from
+      // the correctly rounded quotient by a few ulp of max(|t|, 1); outside a
+      // guard band around the mask edges (ph = 0 and ph = dc) that cannot
+      // change the decision, inside it the exact quotient is used.
+      double t = x * op.inv_pitch + op.half_dc;
+      double ph = t - floor(t);
+      const double guard = (fabs(t) + 1.0) * 1e-13;
+      if (ph < guard || ph > 1.0 - guard || fabs(ph - op.dc) < guard) {
+        t = x / op.pitch + op.half_dc;
+        ph = t - floor(t);
+      }
+      if (ph < op.dc) { re = op.fa[0]; im = op.fa[1]; }
+      else { re = op.fb[0]; im = op.fb[1]; }
+      break;
+    }
+    case XOP_SCALAR:
+      re = op.fa[0]; im = op.fa[1];
+      break;
+    case XOP_POINT: {
+      const double xd = (double)n - (double)N / 2.0;
+      const double x = xd * op.dx - op.xs;
+      const double r = sqrt(x * x + op.z * op.z);
+      const double a = r * op.kphase;
+      double s, c;
+      sincos(a, &s, &c);
+      const double sr = sqrt(r);
+      re = c / sr; im = s / sr;
+      mul = false;
+      break;
+    }
+    case XOP_ONES:
+      re = 1.0; im = 0.0;
+      mul = false;
+      break;
+    default:
+      return v;
+  }
+  // fold the post factor in double, then cast (factors are cast to the field
+  // precision as big-wave's per-chunk factor array is, optical_element.py:151-169)
+  const double pr = re * op.post[0] - im * op.post[1];
+  const double pi = re * op.post[1] + im * op.post[0];
+  T2 f;
+  f.x = (S)pr;
+  f.y = (S)pi;
+  return mul ? cmul(v, f) : f;
+}
+
+// Grating mask decision at sample n (single chunk, factors folded on the
+// host): true selects fa.  The fast test t = fma(n, m_a, m_b) is within
+// m_guard of big-wave's t = x/pitch + dc/2; a fraction within m_guard of an
+// edge (0, dc or 1) is decided on the exact FP64 path instead.
+__device__ __forceinline__ bool mask_in(const XOp &op, double one_minus_guard, int32_t n) {
+#pragma clang fp contract(off)
+  const double nd = (double)n;
+  const double t = __builtin_fma(nd, op.m_a, op.m_b);
+  double ph = t - floor(t);
+  if (ph < op.m_guard || ph > one_minus_guard || fabs(ph - op.dc) < op.m_guard) {
+    const double x = nd * op.xstep + op.x0;
+    const double te = x / op.pitch + op.half_dc;
+    ph = te - floor(te);
+  }
+  return ph < op.dc;
+}
+
+// ---------------------------------------------------------------------------
+// spectrum operations
+// ---------------------------------------------------------------------------
+
+// (cos, sin)(2 pi t) for t in [-1/2, 1/2] given in FP64 (complex128 fields).
+__device__ __forceinline__ void cis_rev(double t, double *c_out, double *s_out) {
+  sincospi(2.0 * t, s_out, c_out);
+}
+
+// spectrum multiply at FFT bin k: phase in revolutions computed in FP64 and
+// reduced to [-1/2, 1/2] before the sine and cosine (big-wave/propagation.py:124-137).
+// Complex128 fields and the plain-form spectrum kernel of both precisions.
+template <typename T2>
+__device__ __forceinline__ T2 apply_kop(const KOp &op, int64_t k, int64_t N, T2 v) {
+  using S = decltype(v.x);
+  if (k >= op.rs && k < op.re) {
+    T2 z; z.x = 0; z.y = 0;
+    return z;
+  }
+  const int64_t ks = (k < N / 2) ? k : k - N;
+  const double kd = (double)ks;
+  double t = op.alpha_rev * (kd * kd) + op.beta_rev;
+  t -= rint(t);
+  double c, s;
+  cis_rev(t, &c, &s);
+  T2 h;
+  h.x = (S)(c * op.scale[0] - s * op.scale[1]);
+  h.y = (S)(c * op.scale[1] + s * op.scale[0]);
+  return cmul(v, h);
+}
+
+// Complex64 spectrum factor exp(2 pi i t) * sc of the row pass.  u = 256 t
+// is formed in FP64 (k'^2 exact up to 2^53, one rounding in the fma), split
+// as u = j + f with j integer and |f| <= 1/2; exp(2 pi i j / 256) comes from
+// a 256-entry table held in LDS and exp(2 pi i f / 256) from its Taylor
+// terms to third (sine) and fourth (cosine) order, truncation < 3e-12 and
+// 2e-15 for |2 pi f / 256| <= 1.23e-2.  Table entries and products are
+// complex64.
+constexpr int kCisTab = 256;
+
+struct KopF32 {
+  const float2 *tab;     // kCisTab entries exp(2 pi i j / kCisTab) (LDS)
+  double au, bu;         // alpha, beta in 1/kCisTab revolutions
+  float s1, s3, c2, c4, sc;  // sc*w, -sc*w^3/6, -sc*w^2/2, sc*w^4/24, sc (w = phi per unit of f)
+  int rs, rlen;          // cutoff band [rs, rs + rlen)
+  int shl;               // 32 - lgN: k' = (k << shl) >> shl
+};
+
+__device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab, int lgN) {
+  KopF32 p;
+  p.tab = tab;
+  p.au = op.alpha_rev * (double)kCisTab;
+  p.bu = op.beta_rev * (double)kCisTab;
+  const double w = 6.283185307179586 / (double)kCisTab;
+  const double sc = op.scale[0];
+  p.sc = (float)sc;
+  p.s1 = (float)(sc * w);
+  p.s3 = (float)(-sc * w * w * w / 6.0);
+  p.c2 = (float)(-sc * w * w / 2.0);
+  p.c4 = (float)(sc * w * w * w * w / 24.0);
+  p.rs = (int)op.rs;
+  p.rlen = (int)(op.re - op.rs);
+  p.shl = 32 - lgN;
+  return p;
+}
+
+__device__ __forceinline__ float2 kop_f32(const KopF32 &p, int k, float2 v) {
+  const int ks = (int)((uint32_t)k << p.shl) >> p.shl;
+  const double kd = (double)ks;
+  const double u = __builtin_fma(p.au, kd * kd, p.bu);
+  const double j = rint(u);
+  const float f = (float)(u - j);
+  // j mod kCisTab from the low mantissa bits of j + 1.5 * 2^52 (|j| < 2^51)
+  const int idx = __double2loint(j + 6755399441055744.0) & (kCisTab - 1);
+  const float2 T = p.tab[idx];
+  const float f2 = f * f;
+  const float cs = fmaf(f2, fmaf(f2, p.c4, p.c2), p.sc);
+  const float sn = f * fmaf(f2, p.s3, p.s1);
+  float2 h;
+  h.x = T.x * cs - T.y * sn;
+  h.y = T.x * sn + T.y * cs;
+  if ((uint32_t)(k - p.rs) < (uint32_t)p.rlen) { h.x = 0.f; h.y = 0.f; }
+  return cmul(v, h);
+}
+
+// frequency bin of row-pass position j in row rr (digit-reversed groups)
+__device__ __forceinline__ int64_t row_kbase(const Geom &g, int64_t rr) {
+  if (g.ngroups == 3) {
+    const int64_t ka = rr >> g.lgB;
+    const int64_t kb = rr & ((int64_t(1) << g.lgB) - 1);
+    return ka + (kb << g.lgA);
+  }
+  return rr;
+}
+
+// Inter-group twiddles of one thread's 16 elements: tw[m] = W_M^{(q + m*TPF) col},
+// M = L * S, as W_M^{q col} * (W_M^{TPF col})^m with the powers built like
+// twiddle_powers (five split-table reads instead of sixteen).
+template <typename T2, int L>
+__device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col, T2 (&tw)[16]) {
+  const T2 *hi = (const T2 *)g.tw_hi;
+  const T2 *lo = (const T2 *)g.tw_lo;
+  const int64_t mask = (int64_t)L * g.S - 1;
+  constexpr int TPF = L / 16;
+  const int64_t eb = ((int64_t)TPF * col) & mask;
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((int64_t)q * col) & mask);
+  T2 b1 = split_twiddle(hi, lo, g.tw_sh, eb);
+  T2 b2 = split_twiddle(hi, lo, g.tw_sh, (2 * eb) & mask);
+  T2 b4 = split_twiddle(hi, lo, g.tw_sh, (4 * eb) & mask);
+  T2 b8 = split_twiddle(hi, lo, g.tw_sh, (8 * eb) & mask);
+  T2 p[8];
+  p[0].x = 1; p[0].y = 0;
+  p[1] = b1; p[2] = b2; p[3] = cmul(b2, b1);
+  p[4] = b4; p[5] = cmul(b4, b1); p[6] = cmul(b4, b2); p[7] = cmul(b4, p[3]);
+  const T2 q8 = cmul(wq, b8);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    tw[m] = cmul(wq, p[m]);
+    tw[m + 8] = cmul(q8, p[m]);
+  }
+}
+
+}  // namespace rs

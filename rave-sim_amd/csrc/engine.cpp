@@ -162,6 +162,7 @@ Plan::~Plan() {
   for (void *q : fib) if (q) (void)hipFree(q);
   for (void *q : hi) if (q) (void)hipFree(q);
   for (void *q : lo) if (q) (void)hipFree(q);
+  if (cis) (void)hipFree(cis);
 }
 
 void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, int *lgB, int *lgR) {
@@ -257,6 +258,16 @@ std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
     p->lo[t] = make_twiddles(lgM, int64_t(1) << s, 1, f64);
     p->hi[t] = make_twiddles(lgM, int64_t(1) << (lgM - s), int64_t(1) << s, f64);
   }
+  // complex64 row-pass phase table exp(+2 pi i j / 256)
+  if (!f64) {
+    std::vector<long double> re(256), im(256);
+    for (int j = 0; j < 256; ++j) {
+      const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)j / 256.0L;
+      re[j] = cosl(ang);
+      im[j] = sinl(ang);
+    }
+    p->cis = upload_table<float>(re, im);
+  }
   return p;
 }
 
@@ -271,6 +282,7 @@ Geom Plan::geom_row() const {
   g.lgB = lgB;
   g.lgR = lgR;
   g.tw_fib = fib[0];
+  g.cis_tab = cis;
   return g;
 }
 static int32_t line_sharing(int lgL, bool f64) {

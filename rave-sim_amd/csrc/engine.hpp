@@ -66,6 +66,7 @@ struct Plan {
   void *fib[3] = {nullptr, nullptr, nullptr};  // per pass: W_L^j
   void *hi[2] = {nullptr, nullptr}, *lo[2] = {nullptr, nullptr};
   int sh[2] = {0, 0};
+  void *cis = nullptr;  // complex64 plans: exp(2 pi i j / 256), j < 256
   Plan() = default;
   Plan(const Plan &) = delete;
   Plan &operator=(const Plan &) = delete;

@@ -20,6 +20,7 @@ struct Launch {
   LKind kind;
   int lgL = 0;
   bool row = false;
+  int xv = 0;  // fiber passes: XopVariant covering every field's x-space op
   Geom geom{};
   size_t arg_off = 0;  // into FieldArgs / DetArgs arrays
   int nf = 0;
@@ -117,6 +118,33 @@ inline XOp compose(const XOp &a, const XOp &b) {
   throw Error(6, "two x-space masks without a propagation in between");
 }
 
+// Prepare an x-space record for the fused passes: mask and scalar factors get
+// the post factor folded in (the same FP64 products the generic kernel path
+// forms per sample, so the cast factors are identical), and single-chunk
+// masks get the constants of the fast test: t = fma(n, m_a, m_b) differs
+// from big-wave's fl(fl(x)/pitch) + dc/2 by a few ulp of the largest term,
+// (|x0| + 2 N |xstep|) / pitch, so m_guard (1e-13 of that plus 1) bounds it
+// with a wide margin.
+inline void finalize_xop(XOp &x, int64_t N) {
+  if (x.kind != XOP_MASK && x.kind != XOP_SCALAR) return;
+  auto fold = [&](double *f) {
+    const double r = f[0] * x.post[0] - f[1] * x.post[1];
+    const double i = f[0] * x.post[1] + f[1] * x.post[0];
+    f[0] = r;
+    f[1] = i;
+  };
+  fold(x.fa);
+  if (x.kind == XOP_MASK) fold(x.fb);
+  x.post[0] = 1.0;
+  x.post[1] = 0.0;
+  if (x.kind == XOP_MASK && x.single_chunk) {
+    x.m_a = x.xstep * x.inv_pitch;
+    x.m_b = x.x0 * x.inv_pitch + x.half_dc;
+    const double span = (std::fabs(x.x0) + 2.0 * (double)N * std::fabs(x.xstep)) * std::fabs(x.inv_pitch);
+    x.m_guard = 1e-13 * (span + std::fabs(x.half_dc) + 1.0);
+  }
+}
+
 template <typename S> struct Exec {
   Context &ctx;
   const RunConfig &rc;
@@ -139,10 +167,12 @@ template <typename S> struct Exec {
     return off;
   }
 
-  void fiber(int lgL, bool row, const Geom &g, const std::vector<FieldArgs> &a) {
+  void fiber(int lgL, bool row, const Geom &g, std::vector<FieldArgs> a) {
+    for (auto &r : a) finalize_xop(r.x, rc.N);
     Launch l{LKind::Fiber};
     l.lgL = lgL;
     l.row = row;
+    l.xv = xop_variant(a.data(), (int)a.size(), g.top != 0);
     l.geom = g;
     l.nf = (int)a.size();
     l.arg_off = push_args(a);
@@ -455,7 +485,7 @@ template <typename S> struct Exec {
           begin_capture();
           const int cls = l.row ? PC_ROW : (l.geom.top ? PC_TOP : PC_MID);
           t_begin(cls, 2.0 * (double)rc.N * (double)esz * l.nf);
-          hip_check(launch_fiber_pass<S>(l.lgL, l.row, fa + l.arg_off, l.nf, l.geom, st), "fiber pass");
+          hip_check(launch_fiber_pass<S>(l.lgL, l.row, l.xv, fa + l.arg_off, l.nf, l.geom, st), "fiber pass");
           t_end();
           break;
         }

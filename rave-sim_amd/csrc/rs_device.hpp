@@ -39,7 +39,15 @@ struct XOp {
   int64_t chunk;
   // point source (big-wave/propagation.py:141-151)
   double xs, z, kphase;
+  // fast mask test (single chunk), set by finalize_xop on the host:
+  // t ~ fma(n, m_a, m_b) is within m_guard of big-wave's x/pitch + dc/2, and
+  // samples whose fraction lies within m_guard of an edge use the exact path
+  double m_a, m_b, m_guard;
 };
+
+// compile-time x-space variants of a fiber pass: none (middle / row passes),
+// grating mask or scalar with the factors folded on the host, or any kind
+enum XopVariant : int32_t { XV_NONE = 0, XV_MASK = 1, XV_GEN = 2 };
 
 // spectrum multiply in the row pass: exp(2 pi i (alpha k'^2 + beta)) * scale,
 // zero for k in [rs, re) (big-wave/propagation.py:74-138).
@@ -81,6 +89,7 @@ struct Geom {
   const void *tw_hi;
   const void *tw_lo;
   const void *tw_fib;  // W_L^j, j < L
+  const void *cis_tab;  // complex64 plans: exp(2 pi i j / 256), j < 256 (row pass phase)
 };
 
 }  // namespace rs

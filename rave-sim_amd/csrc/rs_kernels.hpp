@@ -10,9 +10,28 @@ namespace rs {
 
 // One digit-group pass over `nf` fields (blockIdx.y = field).  `row` selects
 // the contiguous lowest group; otherwise a strided column group of stride g.S.
+// xv (XopVariant) must cover the x-space ops of every field of the launch
+// (xop_variant below).
 template <typename S>
-hipError_t launch_fiber_pass(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g,
+hipError_t launch_fiber_pass(int lgL, bool row, int xv, const FieldArgs *args, int nf, const Geom &g,
                              hipStream_t st);
+// the same for one compile-time variant (fiber_inst.hip)
+template <typename S, int XV>
+hipError_t launch_fiber_xv(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g, hipStream_t st);
+
+// Smallest variant that runs the x-space ops of host records a[0..nf): none,
+// grating mask / scalar (single chunk), or any kind.
+inline int xop_variant(const FieldArgs *a, int nf, bool top) {
+  if (!top) return XV_NONE;
+  int v = XV_NONE;
+  for (int i = 0; i < nf; ++i) {
+    const XOp &x = a[i].x;
+    if (x.kind == XOP_NONE) continue;
+    const bool simple = x.kind == XOP_SCALAR || (x.kind == XOP_MASK && x.single_chunk);
+    v = simple ? (v == XV_GEN ? XV_GEN : XV_MASK) : XV_GEN;
+  }
+  return v;
+}
 template <typename S> hipError_t launch_xop(const FieldArgs *args, int nf, int64_t N, hipStream_t st);
 template <typename S> hipError_t launch_kop(const FieldArgs *args, int nf, const Geom &g, hipStream_t st);
 template <typename S>

@@ -300,10 +300,12 @@ bool Server::tick() {
     int lgL;
     bool row;
     int top;
+    int xv;
     int64_t S;
     const void *tab;
     bool operator<(const Key &o) const {
-      return std::tie(kind, f64, N, lgL, row, top, S, tab) < std::tie(o.kind, o.f64, o.N, o.lgL, o.row, o.top, o.S, o.tab);
+      return std::tie(kind, f64, N, lgL, row, top, xv, S, tab) <
+             std::tie(o.kind, o.f64, o.N, o.lgL, o.row, o.top, o.xv, o.S, o.tab);
     }
   };
   struct Group {
@@ -325,7 +327,7 @@ bool Server::tick() {
       stats_.merged_jobs++;
       continue;
     }
-    Key k{(int)l.kind, j.f64, j.N, l.lgL, l.row, l.geom.top, l.geom.S,
+    Key k{(int)l.kind, j.f64, j.N, l.lgL, l.row, l.geom.top, l.xv, l.geom.S,
           l.kind == LKind::Detect ? (const void *)j.req->bounds : l.geom.tw_fib};
     Group &g = groups[k];
     g.jobs.push_back(&j);
@@ -371,8 +373,8 @@ bool Server::tick() {
     const bool f64 = kv.first.f64;
     switch (l.kind) {
       case LKind::Fiber:
-        hip_check(f64 ? launch_fiber_pass<double>(l.lgL, l.row, (const FieldArgs *)dp, nf, l.geom, st)
-                      : launch_fiber_pass<float>(l.lgL, l.row, (const FieldArgs *)dp, nf, l.geom, st),
+        hip_check(f64 ? launch_fiber_pass<double>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st)
+                      : launch_fiber_pass<float>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st),
                   "fiber pass");
         break;
       case LKind::Xop:
