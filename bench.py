@@ -72,7 +72,7 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
     N = 2 ** lgN
     ms, zdet, steps = cfg3_members(ravesim, N)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet)
-    tuned = ctx.autotune(N, np.complex64)
+    tuned = ctx.autotune(N, np.complex64, members=len(ms))
     run_batch(ctx, N, 2e-9, ms[:2], dtype=np.complex64, **kw)
     t0 = time.perf_counter()
     for _ in range(reps):

@@ -261,10 +261,13 @@ typedef struct rs_pass_stat {
 } rs_pass_stat;
 
 /* Autotuner (no counterpart in the reference): time every valid digit-group
- * split of an N-point field on a synthetic grating slice and keep the fastest
- * for this context.  out4 (optional) = {groups, lg top, lg middle, lg row};
- * ms_per_slice (optional) = measured time of one slice with it. */
-int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int32_t *out4, double *ms_per_slice);
+ * split of an N-point field on a synthetic grating slice loop run over
+ * `members` fields in lockstep (<= 0: 1; capped by free memory) and keep the
+ * fastest for this context and that batch size; runs with another number of
+ * fields per launch use the entry tuned for the nearest size.
+ * out4 (optional) = {groups, lg top, lg middle, lg row}; ms_per_slice
+ * (optional) = measured time of one slice of one field with it. */
+int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice);
 /* Fix the digit-group split for (N, dtype) (lgR = log2 row length, lgA = log2
  * top column length, -1 = default); persisted tuning results are re-applied
  * this way. */

@@ -1,6 +1,7 @@
 // C-ABI of librsim.so (declarations and reference citations in include/ravesim.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -382,14 +383,14 @@ int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bo
   });
 }
 
-int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int32_t *out4, double *ms_per_slice) {
+int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
   if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
   if (N < 64 || (N & (N - 1)) != 0 || N > (int64_t(1) << 30))
     return fail(RS_ERR_UNSUPPORTED, "N must be a power of two in [2^6, 2^30]");
   return guarded([&] {
-    const rs::Context::Tuned t = rs::autotune(*ctx->ctx, N, dtype == RS_C16);
-    rs::Plan &p = ctx->ctx->plan(N, dtype == RS_C16);
+    const rs::Context::Tuned t = rs::autotune(*ctx->ctx, N, dtype == RS_C16, 3, members);
+    rs::Plan &p = ctx->ctx->plan(N, dtype == RS_C16, -1, -1, t.members);
     if (out4) {
       out4[0] = p.ngroups;
       out4[1] = p.lgA;
@@ -408,12 +409,10 @@ int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
     int ng, a, b, r;
     rs::plan_split(lgN, dtype == RS_C16, lgR, lgA, &ng, &a, &b, &r);  // validates
     auto &tv = ctx->ctx->tuned;
-    for (auto it = tv.begin(); it != tv.end(); ++it)
-      if (it->lgN == lgN && it->f64 == (dtype == RS_C16)) {
-        tv.erase(it);
-        break;
-      }
-    tv.push_back(rs::Context::Tuned{lgN, dtype == RS_C16, lgR, lgA, 0.0});
+    tv.erase(std::remove_if(tv.begin(), tv.end(),
+                            [&](const rs::Context::Tuned &t) { return t.lgN == lgN && t.f64 == (dtype == RS_C16); }),
+             tv.end());
+    tv.push_back(rs::Context::Tuned{lgN, dtype == RS_C16, lgR, lgA, 0.0, 0});
   });
 }
 

@@ -45,10 +45,28 @@ template <bool INV, typename T2> __device__ __forceinline__ T2 mul_mi(T2 a) {
   return r;
 }
 
-// element at a byte offset from a workgroup-uniform pointer: keeps the
-// per-lane part of the address in 32 bits (base in SGPRs + 32-bit offset)
-template <typename T2> __device__ __forceinline__ T2 &at(T2 *p, uint32_t off_bytes) {
-  return *(T2 *)((char *)p + off_bytes);
+// Field memory as global-address-space pointers (global_load / global_store
+// instead of flat), addressed as a workgroup-uniform pointer plus a 32-bit
+// per-lane byte offset.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename S> using nvec2 = S __attribute__((ext_vector_type(2)));
+template <typename S> __device__ __forceinline__ gptr<nvec2<S>> global_ptr(void *p) {
+  return (gptr<nvec2<S>>)p;
+}
+template <typename S>
+__device__ __forceinline__ typename C2T<S>::t gload(gptr<nvec2<S>> p, uint32_t off_bytes) {
+  const nvec2<S> r = *(gptr<nvec2<S>>)((gptr<char>)p + off_bytes);
+  typename C2T<S>::t o;
+  o.x = r.x;
+  o.y = r.y;
+  return o;
+}
+template <typename S>
+__device__ __forceinline__ void gstore(gptr<nvec2<S>> p, uint32_t off_bytes, typename C2T<S>::t v) {
+  nvec2<S> r;
+  r.x = v.x;
+  r.y = v.y;
+  *(gptr<nvec2<S>>)((gptr<char>)p + off_bytes) = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -474,6 +492,34 @@ __device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col
   for (int m = 0; m < 8; ++m) {
     tw[m] = cmul(wq, p[m]);
     tw[m + 8] = cmul(q8, p[m]);
+  }
+}
+
+// v[m] *= tw[m] (conj(tw[m]) when CONJ) with the twiddles of inter_twiddles
+// formed on the fly: only the eight powers p[m] stay live (registers)
+template <bool CONJ, typename T2, int L>
+__device__ __forceinline__ void apply_inter_twiddles(const Geom &g, int q, int64_t col, T2 (&v)[16]) {
+  const T2 *hi = (const T2 *)g.tw_hi;
+  const T2 *lo = (const T2 *)g.tw_lo;
+  const int64_t mask = (int64_t)L * g.S - 1;
+  constexpr int TPF = L / 16;
+  const int64_t eb = ((int64_t)TPF * col) & mask;
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((int64_t)q * col) & mask);
+  T2 b1 = split_twiddle(hi, lo, g.tw_sh, eb);
+  T2 b2 = split_twiddle(hi, lo, g.tw_sh, (2 * eb) & mask);
+  T2 b4 = split_twiddle(hi, lo, g.tw_sh, (4 * eb) & mask);
+  T2 b8 = split_twiddle(hi, lo, g.tw_sh, (8 * eb) & mask);
+  T2 p[8];
+  p[0].x = 1; p[0].y = 0;
+  p[1] = b1; p[2] = b2; p[3] = cmul(b2, b1);
+  p[4] = b4; p[5] = cmul(b4, b1); p[6] = cmul(b4, b2); p[7] = cmul(b4, p[3]);
+  const T2 q8 = cmul(wq, b8);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const T2 t0 = cmul(wq, p[m]);
+    const T2 t1 = cmul(q8, p[m]);
+    v[m] = CONJ ? cmulc(v[m], t0) : cmul(v[m], t0);
+    v[m + 8] = CONJ ? cmulc(v[m + 8], t1) : cmul(v[m + 8], t1);
   }
 }
 

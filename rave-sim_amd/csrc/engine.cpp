@@ -329,17 +329,26 @@ Context::~Context() {
   plans_.clear();
   if (stream) (void)hipStreamDestroy(stream);
 }
-const Context::Tuned *Context::find_tuned(int lgN, bool f64) const {
-  for (const auto &t : tuned)
-    if (t.lgN == lgN && t.f64 == f64) return &t;
-  return nullptr;
+const Context::Tuned *Context::find_tuned(int lgN, bool f64, int members) const {
+  const Tuned *best = nullptr;
+  double bd = 1e300;
+  for (const auto &t : tuned) {
+    if (t.lgN != lgN || t.f64 != f64) continue;
+    if (t.members == 0) return &t;  // fixed split
+    const double d = std::fabs(std::log2((double)t.members) - std::log2((double)std::max(members, 1)));
+    if (d < bd) {
+      bd = d;
+      best = &t;
+    }
+  }
+  return best;
 }
 
-Plan &Context::plan(int64_t N, bool f64, int lgR, int lgA) {
+Plan &Context::plan(int64_t N, bool f64, int lgR, int lgA, int members) {
   if (lgR < 0 && lgA < 0) {
     int lgN = 0;
     while ((int64_t(1) << lgN) < N) ++lgN;
-    if (const Tuned *t = find_tuned(lgN, f64)) {
+    if (const Tuned *t = find_tuned(lgN, f64, members)) {
       lgR = t->lgR;
       lgA = t->lgA;
     }
@@ -366,7 +375,6 @@ using namespace exec;
 template <typename S>
 std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   hip_check(hipSetDevice(ctx.device), "hipSetDevice");
-  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA);
   const int64_t npx = detector_pixels(rc.det_size, rc.det_px);
   const std::vector<int64_t> hb = detector_bounds(rc.N, rc.dx, rc.det_size, rc.det_px);
   DevBuf dbounds(hb.size() * sizeof(int64_t));
@@ -391,6 +399,7 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   }
   if (rc.fields) group = (int)members.size();
   group = std::max(1, std::min(group, 65535));
+  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, std::min<int>(group, (int)members.size()));
 
   for (size_t g0 = 0; g0 < members.size(); g0 += group) {
     const size_t g1 = std::min(members.size(), g0 + group);
@@ -439,9 +448,18 @@ std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<M
   return rc.f64 ? run_members_t<double>(ctx, rc, members) : run_members_t<float>(ctx, rc, members);
 }
 
-Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps) {
+Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members) {
   int lgN = 0;
   while ((int64_t(1) << lgN) < N) ++lgN;
+  members = std::max(members, 1);
+  {
+    // all members resident at once: stay within half of the free memory
+    size_t freeb = 0, totalb = 0;
+    hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+    hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
+    const size_t per = (size_t)N * (f64 ? 16 : 8);
+    members = (int)std::max<size_t>(1, std::min<size_t>((size_t)members, freeb / 2 / per));
+  }
   // synthetic slice loop: plane wave, `reps` grating slices of a 4.2 um
   // grating at 46 keV, 1 mm detector
   const double dx = 2e-9, wl = energy_to_wavelength(46000.0);
@@ -479,30 +497,32 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps) {
   rc.dx = dx;
   rc.f64 = f64;
   rc.n_out = 1;
+  rc.max_resident = members;
   const bool was = ctx.timing;
-  Context::Tuned best{lgN, f64, -1, -1, 1e300};
+  Context::Tuned best{lgN, f64, -1, -1, 1e300, members};
   for (auto cand : plan_candidates(lgN, f64)) {
     rc.lgR = cand.first;
     rc.lgA = cand.second;
-    std::vector<Member> ms{m};
+    std::vector<Member> ms(members, m);
     ctx.timing = false;
     (void)run_members(ctx, rc, ms);  // warm: plan tables, code objects
-    ms = {m};
+    ms.assign(members, m);
     for (auto &st : ctx.stats) st = PassStat{};
     ctx.timing = true;
     (void)run_members(ctx, rc, ms);
     double tot = 0;
     for (auto &st : ctx.stats) tot += st.ms;
-    const double per = tot / (reps + 1);
-    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per};
+    const double per = tot / (reps + 1) / members;
+    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, members};
   }
   ctx.timing = was;
   for (auto &st : ctx.stats) st = PassStat{};
   bool replaced = false;
   for (auto &t : ctx.tuned)
-    if (t.lgN == lgN && t.f64 == f64) {
+    if (t.lgN == lgN && t.f64 == f64 && (t.members == members || t.members == 0)) {
       t = best;
       replaced = true;
+      break;
     }
   if (!replaced) ctx.tuned.push_back(best);
   return best;

@@ -146,27 +146,31 @@ class Context {
   hipStream_t stream = nullptr;
   bool timing = false;
   PassStat stats[PC_COUNT];
-  // plan for (N, precision); explicit lgR/lgA, else the tuned split, else default
-  Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1);
+  // plan for (N, precision); explicit lgR/lgA, else the split tuned for the
+  // nearest number of fields per launch, else the default split
+  Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1, int members = 1);
   void sync();
-  // autotuner results: (lgN, f64) -> (lgR, lgA, ms per slice)
+  // autotuner results: (lgN, f64, fields per launch) -> (lgR, lgA, ms per
+  // slice and field); members == 0 marks a fixed split for every batch size
   struct Tuned {
     int lgN;
     bool f64;
     int lgR, lgA;
     double ms;
+    int members;
   };
   std::vector<Tuned> tuned;
-  const Tuned *find_tuned(int lgN, bool f64) const;
+  const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
 
  private:
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;
 };
 
-// Time every candidate split of N on a synthetic grating slice (mask +
-// propagation) and record the fastest in ctx.tuned.  Returns it.
-Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3);
+// Time every candidate split of N on a synthetic grating slice loop (mask +
+// propagation) run over `members` fields in lockstep (one launch per pass
+// covers all of them), and record the fastest in ctx.tuned.  Returns it.
+Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3, int members = 1);
 
 // device buffer RAII
 struct DevBuf {

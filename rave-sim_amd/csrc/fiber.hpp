@@ -1,7 +1,8 @@
 // Fiber pass: one digit-group pass of the four-step FFT over C fibers of
-// length L per workgroup, staged through LDS with radix-16 Stockham stages,
-// with the elementwise work of the split step fused in (x-space multiply in
-// the top column pass, spectrum multiply between the row DFT and row iDFT).
+// length L per workgroup tile, staged through LDS with radix-16 Stockham
+// stages, with the elementwise work of the split step fused in (x-space
+// multiply in the top column pass, spectrum multiply between the row DFT and
+// row iDFT).
 //
 // Compile-time variants: S (complex64 / complex128 fields), L and C, ROW
 // (contiguous lowest group or strided column group) and XV (x-space op: none,
@@ -42,119 +43,143 @@ __global__ void __launch_bounds__(C *(L / 16))
   const int q = ROW ? t % TPF : t / C;
   T2 *fib = lds + c * FIB;
 
-  // sample index of v[m]: nb + lane + m * mst, with nb and mst uniform over
-  // the workgroup and lane * sizeof(T2) < 2^31 for N <= 2^30
-  int64_t nb, mst, col = 0, rr = 0;
-  uint32_t lane;
-  if (ROW) {
-    rr = (int64_t)blockIdx.x * C + c;
-    nb = (int64_t)blockIdx.x * C * L;
-    lane = (uint32_t)(c * L + q);
-    mst = TPF;
-  } else {
+  // Sample index of v[m] in the workgroup's tile: nb + lane + m * mst, with
+  // nb and mst uniform over the workgroup and lane * sizeof(T2) < 2^31 for
+  // N <= 2^30.
+  const int64_t ntiles = g.N / ((int64_t)L * C);
+  const int64_t mst = ROW ? (int64_t)TPF : (int64_t)TPF * g.S;
+  const uint32_t lane = ROW ? (uint32_t)(c * L + q) : (uint32_t)(c + (int64_t)q * g.S);
+  const uint32_t lb = lane * (uint32_t)sizeof(T2);
+  const int64_t cpb = ROW ? 1 : g.S / C;
+  // first sample (nb) and first column of a tile
+  auto tile_base = [&](int64_t tile, int64_t &nb, int64_t &c0) {
+    if (ROW) {
+      nb = tile * C * L;
+      c0 = 0;
+      return;
+    }
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; when a
     // row segment of C columns is shorter than a 128 B line, the `segs`
-    // workgroups that share one line are given ids on the same XCD so that
-    // its L2 merges their partial-line reads and writes
-    int64_t bid = blockIdx.x;
+    // tiles that share one line are given ids on the same XCD so that its L2
+    // merges their partial-line reads and writes
+    int64_t bid = tile;
     if (g.xcd_segs > 1) {
       const int64_t G = 8 * (int64_t)g.xcd_segs;
-      if (bid < ((int64_t)gridDim.x / G) * G) {
+      if (bid < (ntiles / G) * G) {
         const int64_t r = bid % G;
         bid = (bid - r) + (r % 8) * g.xcd_segs + r / 8;
       }
     }
-    const int64_t cpb = g.S / C;
     const int64_t hi = bid / cpb;
-    const int64_t c0 = (bid % cpb) * C;
+    c0 = (bid % cpb) * C;
     nb = hi * (int64_t)L * g.S + c0;
-    col = c0 + c;
-    lane = (uint32_t)(c + (int64_t)q * g.S);
-    mst = (int64_t)TPF * g.S;
-  }
-  T2 *__restrict__ p0 = (T2 *)A.data + nb;
-  const uint32_t lb = lane * (uint32_t)sizeof(T2);
+  };
 
+  const gptr<nvec2<S>> data = global_ptr<S>(A.data);
   const bool xop = XV != XV_NONE && g.top && A.x.kind != XOP_NONE;
   bool gen = false;
   if constexpr (XV == XV_MASK) gen = xop && A.x.from_ones;
   if constexpr (XV == XV_GEN) gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones);
+
+  const int64_t tile = blockIdx.x;
+  int64_t nb, c0;
+  tile_base(tile, nb, c0);
   T2 v[16];
   if (!gen) {
+    const gptr<nvec2<S>> p0 = data + nb;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = at(p0 + m * mst, lb);
+    for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb);
   } else {
 #pragma unroll
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
+  {
+    const int64_t col = c0 + c;
+    const int64_t rr = tile * C + c;
 
-  if (!ROW && A.do_inv) {
+    // inter-group twiddles: top fibers up to 256 keep them live from the
+    // inverse to the forward DFT; otherwise they are recomputed (registers)
+    constexpr bool KEEP_TW = !ROW && L <= 256 && XV != XV_NONE;
     T2 tw[16];
-    inter_twiddles<T2, L>(g, q, col, tw);
+    bool have_tw = false;
+    if (!ROW && A.do_inv) {
+      if constexpr (KEEP_TW) {
+        inter_twiddles<T2, L>(g, q, col, tw);
+        have_tw = true;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
-    stockham<T2, L, 1, true>(v, fib, q, twf);
-  }
+        for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
+      } else {
+        apply_inter_twiddles<true, T2, L>(g, q, col, v);
+      }
+      stockham<T2, L, 1, true>(v, fib, q, twf);
+    }
 
-  if constexpr (XV == XV_MASK) {
-    if (xop) {
-      const XOp &x = A.x;
-      T2 fa, fb;
-      fa.x = (S)x.fa[0]; fa.y = (S)x.fa[1];
-      fb.x = (S)x.fb[0]; fb.y = (S)x.fb[1];
-      if (x.kind == XOP_MASK) {
-        const double omg = 1.0 - x.m_guard;
-        const int32_t n0 = (int32_t)(nb + lane);
-        const int32_t ms = (int32_t)mst;
+    if constexpr (XV == XV_MASK) {
+      if (xop) {
+        const XOp &x = A.x;
+        T2 fa, fb;
+        fa.x = (S)x.fa[0]; fa.y = (S)x.fa[1];
+        fb.x = (S)x.fb[0]; fb.y = (S)x.fb[1];
+        if (x.kind == XOP_MASK) {
+          const double omg = 1.0 - x.m_guard;
+          const int32_t n0 = (int32_t)(nb + lane);
+          const int32_t ms = (int32_t)mst;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-          const T2 f = mask_in(x, omg, n0 + m * ms) ? fa : fb;
-          v[m] = gen ? f : cmul(v[m], f);
+          for (int m = 0; m < 16; ++m) {
+            const T2 f = mask_in(x, omg, n0 + m * ms) ? fa : fb;
+            v[m] = gen ? f : cmul(v[m], f);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < 16; ++m) v[m] = gen ? fa : cmul(v[m], fa);
         }
+      }
+    } else if constexpr (XV == XV_GEN) {
+      if (xop) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = apply_xop(A.x, nb + lane + m * mst, g.N, v[m]);
+      }
+    }
+
+    if (A.do_fwd) {
+      stockham<T2, L, 1, false>(v, fib, q, twf);
+      if (!ROW) {
+        if (have_tw) {
+#pragma unroll
+          for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
+        } else {
+          apply_inter_twiddles<false, T2, L>(g, q, col, v);
+        }
+      }
+    }
+
+    if (ROW && A.k.on) {
+      const int64_t kb = row_kbase(g, rr);
+      const int64_t kstep = g.N >> g.lgR;
+      if constexpr (sizeof(S) == 4) {
+        const KopF32 kp = kop_f32_setup(A.k, ktab, 63 - __clzll((unsigned long long)g.N));
+        const int k0 = (int)(kb + kstep * q);
+        const int kst = (int)(kstep * TPF);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = kop_f32(kp, k0 + m * kst, v[m]);
       } else {
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = gen ? fa : cmul(v[m], fa);
+        for (int m = 0; m < 16; ++m) {
+          const int64_t k = kb + kstep * (q + m * TPF);
+          v[m] = apply_kop(A.k, k, g.N, v[m]);
+        }
       }
     }
-  } else if constexpr (XV == XV_GEN) {
-    if (xop) {
+
+    if (ROW && A.do_inv) stockham<T2, L, 1, true>(v, fib, q, twf);
+
+    {
+      const gptr<nvec2<S>> p0 = data + nb;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = apply_xop(A.x, nb + lane + m * mst, g.N, v[m]);
+      for (int m = 0; m < 16; ++m) gstore<S>(p0 + m * mst, lb, v[m]);
     }
+
   }
-
-  if (A.do_fwd) {
-    stockham<T2, L, 1, false>(v, fib, q, twf);
-    if (!ROW) {
-      T2 tw[16];  // recomputed rather than kept live across the FFTs (registers)
-      inter_twiddles<T2, L>(g, q, col, tw);
-#pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
-    }
-  }
-
-  if (ROW && A.k.on) {
-    const int64_t kb = row_kbase(g, rr);
-    const int64_t kstep = g.N >> g.lgR;
-    if constexpr (sizeof(S) == 4) {
-      const KopF32 kp = kop_f32_setup(A.k, ktab, 63 - __clzll((unsigned long long)g.N));
-      const int k0 = (int)(kb + kstep * q);
-      const int kst = (int)(kstep * TPF);
-#pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = kop_f32(kp, k0 + m * kst, v[m]);
-    } else {
-#pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const int64_t k = kb + kstep * (q + m * TPF);
-        v[m] = apply_kop(A.k, k, g.N, v[m]);
-      }
-    }
-  }
-
-  if (ROW && A.do_inv) stockham<T2, L, 1, true>(v, fib, q, twf);
-
-#pragma unroll
-  for (int m = 0; m < 16; ++m) at(p0 + m * mst, lb) = v[m];
 }
 
 // columns per workgroup for a column pass of fiber length L: 16 columns of
@@ -169,13 +194,21 @@ static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipSt
   return hipGetLastError();
 }
 
+// rows per workgroup: full waves and 256 threads (complex64) / 128 threads
+// (complex128) for rows up to 4096 / 2048 samples, one row beyond; fields
+// with fewer rows than that (single-group fields) use one row per workgroup
 template <typename S, int L, int XV>
 static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
-  constexpr int C = L >= 256 ? 1 : 256 / L;
+  constexpr int SPAN = sizeof(S) == 8 ? 2048 : 4096;
+  constexpr int C = L >= SPAN ? 1 : SPAN / L;
   const int64_t rows = g.N / L;
-  if (rows % C != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)(rows / C), (unsigned)nf);
-  hipLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, args, g);
+  if (rows % C == 0) {
+    dim3 grid((unsigned)(rows / C), (unsigned)nf);
+    hipLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, args, g);
+  } else {
+    dim3 grid((unsigned)rows, (unsigned)nf);
+    hipLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, args, g);
+  }
   return hipGetLastError();
 }
 

@@ -16,6 +16,9 @@ double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// fields per launch assumed when picking a tuned split for served jobs
+constexpr int kServeBatchHint = 16;
+
 }  // namespace
 
 // device / pinned buffers reused across jobs (hipFree would stall the stream)
@@ -215,7 +218,9 @@ template <typename S> void Server::compile_job(Job &j) {
   rc.det_py = c.py;
   rc.z_det = c.z_detector;
   rc.n_out = (int)j.req->res.n_steps;
-  Plan &plan = ctx_.plan(c.N, c.f64);
+  // jobs of one size are merged into shared launches: use the split tuned
+  // for batched launches when there is one
+  Plan &plan = ctx_.plan(c.N, c.f64, -1, -1, kServeBatchHint);
   exec::Exec<S> ex(ctx_, rc, plan);
   ex.mem = {&j.m};
   ex.field = {j.field};

@@ -124,7 +124,7 @@ def _load() -> C.CDLL:
                                          C.POINTER(C.c_int64), C.c_int64]),
         "rs_batch_run": (C.c_int, [P, C.POINTER(rs_batch_desc), C.POINTER(C.c_double), C.c_int64]),
         "rs_set_timing": (C.c_int, [P, C.c_int]),
-        "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+        "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "rs_set_split": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.c_int]),
         "rs_server_create": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(P)]),
         "rs_server_submit": (C.c_int, [P, C.POINTER(rs_request), C.POINTER(C.c_int64)]),

@@ -42,12 +42,13 @@ class Context:
         """Per-launch HIP-event timing of the fused passes (resets counters)."""
         check(lib.rs_set_timing(self.handle, 1 if enable else 0))
 
-    def autotune(self, N: int, dtype=np.complex64) -> dict:
-        """Pick the fastest digit-group split for (N, dtype) on this device."""
+    def autotune(self, N: int, dtype=np.complex64, members: int = 1) -> dict:
+        """Pick the fastest digit-group split for (N, dtype) on this device,
+        timed with `members` fields per launch (the batch size it will serve)."""
         out = (C.c_int32 * 4)()
         ms = C.c_double()
         code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16
-        check(lib.rs_autotune(self.handle, int(N), code, out, C.byref(ms)))
+        check(lib.rs_autotune(self.handle, int(N), code, int(members), out, C.byref(ms)))
         return {"groups": out[0], "lgA": out[1], "lgB": out[2], "lgR": out[3], "ms_per_slice": ms.value}
 
     def set_split(self, N: int, dtype, lgR: int, lgA: int = -1) -> None:

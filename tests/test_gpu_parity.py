@@ -210,3 +210,31 @@ def test_forced_splits_match_oracle(N, dtype, lgR, lgA):
     assert rel_l2(got, ref.u) < tol
     f.close()
     c.close()
+
+
+def test_autotune_per_batch_size():
+    """Splits tuned for one field and for a lockstep batch are both valid
+    plans; a batch run under the batch-tuned split matches the default
+    split's result (complex64 rounding only)."""
+    import ravesim
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 20, 2e-9
+    c = ravesim.Context(0)
+    one = c.autotune(N, np.complex64, members=1)
+    many = c.autotune(N, np.complex64, members=4)
+    for t in (one, many):
+        assert t["groups"] in (1, 2, 3) and t["ms_per_slice"] > 0
+        assert t["lgA"] + t["lgB"] + t["lgR"] == 20 or (t["groups"] == 1 and t["lgR"] == 20)
+    ms = []
+    for k in range(4):
+        g = ravesim.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=30e-6, nr_steps=4,
+                            db_a=complex(7.7e-7 / (1 + k), 3.5e-10))
+        ms.append(BatchMember(wl=ravesim.energy_to_wavelength((30 + 5 * k) * 1e3), gratings=[g],
+                              cutoff_angles=[math.pi / 2] * 2, weight=1.0 + k, plane_wave=True))
+    kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=0.05)
+    tuned = run_batch(c, N, dx, ms, **kw)
+    c.close()
+    d = ravesim.Context(0)
+    default = run_batch(d, N, dx, ms, **kw)
+    d.close()
+    assert rel_l2(tuned, default) < 1e-5
