@@ -194,7 +194,8 @@ typedef struct rs_batch_desc {
   int32_t nr_members;
   const rs_batch_member *members;
   int32_t nr_outputs;      /* accumulator rows */
-  int32_t max_resident;    /* members resident at once (0 = as many as memory allows) */
+  int32_t max_resident;    /* members resident at once, one launch per pass covering them; 0 = as many
+                              as fit the 256 MiB last-level cache together (at least one), bounded by memory */
   int32_t use_graph;       /* capture each group's launch sequence in a hipGraph */
   int32_t reserved;
 } rs_batch_desc;
@@ -262,9 +263,9 @@ typedef struct rs_pass_stat {
 
 /* Autotuner (no counterpart in the reference): time every valid digit-group
  * split of an N-point field on a synthetic grating slice loop run over
- * `members` fields in lockstep (<= 0: 1; capped by free memory) and keep the
- * fastest for this context and that batch size; runs with another number of
- * fields per launch use the entry tuned for the nearest size.
+ * `members` fields (<= 0: 1) grouped as rs_batch_run groups them by default,
+ * and keep the fastest for this context and that launch width (fields per
+ * launch); runs with another width use the entry tuned for the nearest one.
  * out4 (optional) = {groups, lg top, lg middle, lg row}; ms_per_slice
  * (optional) = measured time of one slice of one field with it. */
 int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice);

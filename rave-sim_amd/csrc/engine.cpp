@@ -372,6 +372,18 @@ void Context::sync() {
 namespace {
 using namespace exec;
 
+// Default lockstep group: the members whose buffers fit the device's
+// last-level cache (MALL) together, at least one, so that a group runs all
+// of its passes out of the cache; bounded by free memory (minus 2 GiB).
+int default_group(size_t bytes_per_member, int64_t n_members) {
+  size_t freeb = 0, totalb = 0;
+  hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
+  const size_t budget = freeb > (size_t)(2ull << 30) ? freeb - (size_t)(2ull << 30) : freeb / 2;
+  const size_t cached = std::max<size_t>(1, kMallBytes / bytes_per_member);
+  const size_t g = std::min<size_t>({(size_t)n_members, budget / bytes_per_member, cached});
+  return (int)std::max<size_t>(1, std::min<size_t>(g, 65535));
+}
+
 template <typename S>
 std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   hip_check(hipSetDevice(ctx.device), "hipSetDevice");
@@ -390,13 +402,8 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
 
   const size_t fbytes = (size_t)rc.N * 2 * sizeof(S);
   int group = rc.max_resident > 0 ? rc.max_resident : (int)members.size();
-  if (rc.max_resident <= 0 && !rc.fields) {
-    size_t freeb = 0, totalb = 0;
-    hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
-    const size_t per = fbytes * (needs_snap ? 2 : 1);
-    const size_t budget = freeb > (size_t)(2ull << 30) ? freeb - (size_t)(2ull << 30) : freeb / 2;
-    group = (int)std::max<size_t>(1, std::min<size_t>(members.size(), budget / per));
-  }
+  if (rc.max_resident <= 0 && !rc.fields)
+    group = default_group(fbytes * (needs_snap ? 2 : 1), (int64_t)members.size());
   if (rc.fields) group = (int)members.size();
   group = std::max(1, std::min(group, 65535));
   Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, std::min<int>(group, (int)members.size()));
@@ -497,9 +504,13 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members
   rc.dx = dx;
   rc.f64 = f64;
   rc.n_out = 1;
-  rc.max_resident = members;
+  // fields per launch as the lockstep runner will group `members` fields;
+  // time a few groups of that width
+  const int width = default_group((size_t)N * (f64 ? 16 : 8), members);
+  members = std::min(members, 4 * width);
+  rc.max_resident = width;
   const bool was = ctx.timing;
-  Context::Tuned best{lgN, f64, -1, -1, 1e300, members};
+  Context::Tuned best{lgN, f64, -1, -1, 1e300, width};
   for (auto cand : plan_candidates(lgN, f64)) {
     rc.lgR = cand.first;
     rc.lgA = cand.second;
@@ -513,13 +524,13 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members
     double tot = 0;
     for (auto &st : ctx.stats) tot += st.ms;
     const double per = tot / (reps + 1) / members;
-    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, members};
+    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, width};
   }
   ctx.timing = was;
   for (auto &st : ctx.stats) st = PassStat{};
   bool replaced = false;
   for (auto &t : ctx.tuned)
-    if (t.lgN == lgN && t.f64 == f64 && (t.members == members || t.members == 0)) {
+    if (t.lgN == lgN && t.f64 == f64 && (t.members == width || t.members == 0)) {
       t = best;
       replaced = true;
       break;

@@ -21,6 +21,10 @@
 
 namespace rs {
 
+// MI355X last-level cache (AMD Infinity Cache / MALL) per device: default
+// lockstep groups keep their fields within it
+constexpr size_t kMallBytes = size_t(256) << 20;
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
