@@ -408,28 +408,55 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   group = std::max(1, std::min(group, 65535));
   Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, std::min<int>(group, (int)members.size()));
 
+  // Field (and snapshot) buffers of one group, reused by every group: each
+  // member program starts by generating or uploading its field.  All groups
+  // are compiled first, their argument records go to the device in one copy,
+  // and the launches of all groups are enqueued back to back.
+  std::vector<DevBuf> fbuf, sbuf;
+  const size_t width = std::min(members.size(), (size_t)group);
+  if (!rc.fields)
+    for (size_t i = 0; i < width; ++i) fbuf.emplace_back(fbytes);
+  if (needs_snap)
+    for (size_t i = 0; i < width; ++i) sbuf.emplace_back(fbytes);
+  std::vector<std::unique_ptr<Exec<S>>> groups;
+  size_t nfa = 0, nda = 0;
   for (size_t g0 = 0; g0 < members.size(); g0 += group) {
     const size_t g1 = std::min(members.size(), g0 + group);
-    Exec<S> ex(ctx, rc, plan);
-    std::vector<DevBuf> fbuf, sbuf;
+    auto ex = std::make_unique<Exec<S>>(ctx, rc, plan);
     for (size_t i = g0; i < g1; ++i) {
-      ex.mem.push_back(&members[i]);
-      if (rc.fields) {
-        ex.field.push_back(rc.fields[i]);
-      } else {
-        fbuf.emplace_back(fbytes);
-        ex.field.push_back(fbuf.back().p);
-      }
-      if (needs_snap) {
-        sbuf.emplace_back(fbytes);
-        ex.snap.push_back(sbuf.back().p);
-      }
+      ex->mem.push_back(&members[i]);
+      ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[i - g0].p);
+      if (needs_snap) ex->snap.push_back(sbuf[i - g0].p);
     }
-    ex.acc = (double *)dacc.p;
-    ex.bounds = (const int64_t *)dbounds.p;
-    ex.npx = npx;
-    ex.compile();
-    ex.execute();
+    ex->acc = (double *)dacc.p;
+    ex->bounds = (const int64_t *)dbounds.p;
+    ex->npx = npx;
+    ex->compile();
+    nfa += ex->fargs.size();
+    nda += ex->dargs.size();
+    groups.push_back(std::move(ex));
+  }
+  std::vector<FieldArgs> hfa;
+  std::vector<DetArgs> hda;
+  hfa.reserve(nfa);
+  hda.reserve(nda);
+  for (auto &ex : groups) {
+    hfa.insert(hfa.end(), ex->fargs.begin(), ex->fargs.end());
+    hda.insert(hda.end(), ex->dargs.begin(), ex->dargs.end());
+  }
+  DevBuf dfa(std::max<size_t>(nfa, 1) * sizeof(FieldArgs));
+  DevBuf dda(std::max<size_t>(nda, 1) * sizeof(DetArgs));
+  if (nfa)
+    hip_check(hipMemcpyAsync(dfa.p, hfa.data(), nfa * sizeof(FieldArgs), hipMemcpyHostToDevice, ctx.stream),
+              "args upload");
+  if (nda)
+    hip_check(hipMemcpyAsync(dda.p, hda.data(), nda * sizeof(DetArgs), hipMemcpyHostToDevice, ctx.stream),
+              "det args upload");
+  size_t ofa = 0, oda = 0;
+  for (auto &ex : groups) {
+    ex->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
+    ofa += ex->fargs.size();
+    oda += ex->dargs.size();
   }
 
   std::vector<double> accs(nacc);
@@ -437,6 +464,7 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     hip_check(hipMemcpyAsync(accs.data(), dacc.p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
               "acc download");
   hip_check(hipStreamSynchronize(ctx.stream), "sync");
+  for (auto &ex : groups) ex->collect_timing();
 
   const std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
   std::vector<double> out(nacc);

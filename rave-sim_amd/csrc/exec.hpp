@@ -424,6 +424,11 @@ template <typename S> struct Exec {
     }
   }
 
+  // per-launch event timing records of the enqueued launches
+  std::vector<hipEvent_t> ev;
+  std::vector<std::pair<int, double>> ev_meta;  // class, bytes
+
+  // Upload this program's argument records, enqueue it and wait for it.
   void execute() {
     hipStream_t st = ctx.stream;
     DevBuf dfa(std::max<size_t>(fargs.size(), 1) * sizeof(FieldArgs));
@@ -434,8 +439,16 @@ template <typename S> struct Exec {
     if (!dargs.empty())
       hip_check(hipMemcpyAsync(dda.p, dargs.data(), dargs.size() * sizeof(DetArgs), hipMemcpyHostToDevice, st),
                 "det args upload");
-    const FieldArgs *fa = (const FieldArgs *)dfa.p;
-    const DetArgs *da = (const DetArgs *)dda.p;
+    enqueue((const FieldArgs *)dfa.p, (const DetArgs *)dda.p);
+    hip_check(hipStreamSynchronize(st), "run sync");
+    collect_timing();
+  }
+
+  // Enqueue the launches on the context stream with argument records already
+  // on the device (fa[i] = fargs[i], da[i] = dargs[i]); returns without
+  // waiting unless a graph or a host download needs it.
+  void enqueue(const FieldArgs *fa, const DetArgs *da) {
+    hipStream_t st = ctx.stream;
     const size_t esz = 2 * sizeof(S);
 
     // graph capture covers the run of launches between host interactions
@@ -463,8 +476,6 @@ template <typename S> struct Exec {
 
     // optional per-launch event timing (not combined with graph capture)
     const bool timed = ctx.timing && !rc.use_graph;
-    std::vector<hipEvent_t> ev;
-    std::vector<std::pair<int, double>> ev_meta;  // class, bytes
     auto t_begin = [&](int cls, double bytes) {
       if (!timed) return;
       hipEvent_t a, b;
@@ -533,7 +544,10 @@ template <typename S> struct Exec {
       }
     }
     end_capture();
-    hip_check(hipStreamSynchronize(st), "run sync");
+  }
+
+  // after the stream has drained: add the timed launches to ctx.stats
+  void collect_timing() {
     for (size_t i = 0; i < ev_meta.size(); ++i) {
       float ms = 0;
       hip_check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "event elapsed");
@@ -543,6 +557,8 @@ template <typename S> struct Exec {
       p.bytes += ev_meta[i].second;
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    ev_meta.clear();
   }
 };
 
