@@ -13,6 +13,7 @@
 #include <cstdint>
 
 #include "rs_device.hpp"
+#include "rs_kernels.hpp"
 
 namespace rs {
 
@@ -214,7 +215,7 @@ __device__ __forceinline__ T2 split_twiddle(const T2 *__restrict__ hi, const T2 
 
 template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV>
 __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
-                                               const T2 *__restrict__ twf) {
+                                               const T2 *__restrict__ twf, const T2 *stab) {
   constexpr int TPF = L / 16;
   constexpr int NB = 16 / R;
   constexpr int LR = L / R;
@@ -230,11 +231,18 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
       const T2 *src = fib + lds_pad(b);
 #pragma unroll
       for (int r = 0; r < R; ++r) x[i][r] = src[lds_step<LR>(r)];
-      // W_{NS R}^{r k} for r < R from a few table reads (k, 2k, 4k, 8k);
-      // the remaining powers are products of at most two of them
-      const int e = (b % NS) * (L / (NS * R));
+      // W_{NS R}^{r k} for r < R: small stages read all of them from their
+      // [k][r] table in LDS; larger ones read k, 2k, 4k, 8k from W_L^j and
+      // form the other powers as products of at most two of them
       T2 w[R];
-      twiddle_powers<T2, R>(twf, e, L - 1, w);
+      if constexpr (NS <= kSmallNS) {
+        const T2 *row = stab + small_tab_offset(L, NS) + (b % NS) * kTwRow;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = row[r];
+      } else {
+        const int e = (b % NS) * (L / (NS * R));
+        twiddle_powers<T2, R>(twf, e, L - 1, w);
+      }
 #pragma unroll
       for (int r = 1; r < R; ++r) x[i][r] = INV ? cmulc(x[i][r], w[r]) : cmul(x[i][r], w[r]);
     }
@@ -271,12 +279,13 @@ template <int L, int NS> constexpr int stage_radix() {
 }
 
 template <typename T2, int L, int NS, bool INV>
-__device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf) {
+__device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf,
+                                         const T2 *stab) {
   constexpr int R = stage_radix<L, NS>();
   constexpr bool FIRST = NS == 1;
   constexpr bool LAST = NS * R == L;
-  stockham_stage<T2, L, R, NS, FIRST, LAST, INV>(v, fib, q, twf);
-  if constexpr (!LAST) stockham<T2, L, NS * R, INV>(v, fib, q, twf);
+  stockham_stage<T2, L, R, NS, FIRST, LAST, INV>(v, fib, q, twf, stab);
+  if constexpr (!LAST) stockham<T2, L, NS * R, INV>(v, fib, q, twf, stab);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,7 +448,7 @@ __device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab
   return p;
 }
 
-__device__ __forceinline__ float2 kop_f32(const KopF32 &p, int k, float2 v) {
+template <bool CUT> __device__ __forceinline__ float2 kop_f32(const KopF32 &p, int k, float2 v) {
   const int ks = (int)((uint32_t)k << p.shl) >> p.shl;
   const double kd = (double)ks;
   const double u = __builtin_fma(p.au, kd * kd, p.bu);
@@ -454,7 +463,7 @@ __device__ __forceinline__ float2 kop_f32(const KopF32 &p, int k, float2 v) {
   float2 h;
   h.x = T.x * cs - T.y * sn;
   h.y = T.x * sn + T.y * cs;
-  if ((uint32_t)(k - p.rs) < (uint32_t)p.rlen) { h.x = 0.f; h.y = 0.f; }
+  if (CUT && (uint32_t)(k - p.rs) < (uint32_t)p.rlen) { h.x = 0.f; h.y = 0.f; }
   return cmul(v, h);
 }
 

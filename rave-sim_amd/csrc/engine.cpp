@@ -237,6 +237,31 @@ static void *make_twiddles(int lgM, int64_t count, int64_t mult, bool f64) {
   return f64 ? upload_table<double>(re, im) : upload_table<float>(re, im);
 }
 
+// In-fiber twiddles of a fiber of length L = 2^lgL: W_L^j for j < L, then
+// the [k][r] tables of the small Stockham stages (small_tab_offset):
+// entry k * kTwRow + r of stage NS is W_{16 NS}^{k r} = W_L^{k r L / (16 NS)}
+// for r < 16, zero in the pad column.
+static void *make_fiber_table(int lgL, bool f64) {
+  const int L = 1 << lgL;
+  const int n = L + small_tab_entries(L);
+  std::vector<long double> re(n, 0.0L), im(n, 0.0L);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  auto w = [&](int64_t e, long double *c, long double *s) {
+    const long double ang = -two_pi * (long double)(e & (L - 1)) / (long double)L;
+    *c = cosl(ang);
+    *s = sinl(ang);
+  };
+  for (int j = 0; j < L; ++j) w(j, &re[j], &im[j]);
+  for (int ns = stage_r0(L); ns < L; ns *= 16) {
+    if (ns <= 1 || ns > kSmallNS) continue;
+    const int base = L + small_tab_offset(L, ns);
+    const int64_t step = L / (16 * ns);
+    for (int k = 0; k < ns; ++k)
+      for (int r = 0; r < 16; ++r) w((int64_t)k * r * step, &re[base + k * kTwRow + r], &im[base + k * kTwRow + r]);
+  }
+  return f64 ? upload_table<double>(re, im) : upload_table<float>(re, im);
+}
+
 std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
   int lgN = 0;
   while ((int64_t(1) << lgN) < N) ++lgN;
@@ -247,9 +272,9 @@ std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
   p->f64 = f64;
   plan_split(lgN, f64, lgR_req, lgA_req, &p->ngroups, &p->lgA, &p->lgB, &p->lgR);
   // in-fiber tables: row, top, middle
-  p->fib[0] = make_twiddles(p->lgR, int64_t(1) << p->lgR, 1, f64);
-  if (p->ngroups >= 2) p->fib[1] = make_twiddles(p->lgA, int64_t(1) << p->lgA, 1, f64);
-  if (p->ngroups == 3) p->fib[2] = make_twiddles(p->lgB, int64_t(1) << p->lgB, 1, f64);
+  p->fib[0] = make_fiber_table(p->lgR, f64);
+  if (p->ngroups >= 2) p->fib[1] = make_fiber_table(p->lgA, f64);
+  if (p->ngroups == 3) p->fib[2] = make_fiber_table(p->lgB, f64);
   // inter-group tables: top column M = N, middle column M = 2^(lgB + lgR)
   for (int t = 0; t < p->ngroups - 1; ++t) {
     const int lgM = t == 0 ? lgN : p->lgB + p->lgR;

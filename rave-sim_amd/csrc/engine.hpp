@@ -67,7 +67,7 @@ struct Plan {
   int ngroups = 0;
   int lgA = 0, lgB = 0, lgR = 0;  // top, middle, row (lgA = lgR when ngroups == 1)
   // device tables (owned)
-  void *fib[3] = {nullptr, nullptr, nullptr};  // per pass: W_L^j
+  void *fib[3] = {nullptr, nullptr, nullptr};  // per pass: W_L^j, then small-stage tables
   void *hi[2] = {nullptr, nullptr}, *lo[2] = {nullptr, nullptr};
   int sh[2] = {0, 0};
   void *cis = nullptr;  // complex64 plans: exp(2 pi i j / 256), j < 256
@@ -118,7 +118,7 @@ struct RunConfig {
   int mode = 0;              // 0 fused, 1 plain
   double det_size = 0, det_px = 0, det_py = 1, z_det = 0;
   int n_out = 1;             // accumulator rows
-  int max_resident = 0;      // 0 = all members at once
+  int max_resident = 0;      // members per group; 0 = as many as fit the last-level cache together
   bool use_graph = false;
   int lgR = -1;              // row length override (autotuner), -1 = tuned or default
   int lgA = -1;              // top column length override
@@ -172,8 +172,9 @@ class Context {
 };
 
 // Time every candidate split of N on a synthetic grating slice loop (mask +
-// propagation) run over `members` fields in lockstep (one launch per pass
-// covers all of them), and record the fastest in ctx.tuned.  Returns it.
+// propagation) run over `members` fields grouped as run_members groups them
+// by default (one launch per pass covers a group), and record the fastest in
+// ctx.tuned under that launch width.  Returns it.
 Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3, int members = 1);
 
 // device buffer RAII

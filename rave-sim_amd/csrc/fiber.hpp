@@ -32,6 +32,12 @@ __global__ void __launch_bounds__(C *(L / 16))
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
   const int t = threadIdx.x;
+  // twiddle tables of the small Stockham stages, staged in LDS (first read
+  // after the first stage's barrier)
+  constexpr int NSTAB = small_tab_entries(L);
+  __shared__ T2 stab[NSTAB > 0 ? NSTAB : 1];
+  if constexpr (NSTAB > 0)
+    for (int i = t; i < NSTAB; i += C * TPF) stab[i] = twf[L + i];
   if constexpr (KTAB) {
     if (A.k.on) {
       const float2 *gt = (const float2 *)g.cis_tab;
@@ -111,7 +117,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       } else {
         apply_inter_twiddles<true, T2, L>(g, q, col, v);
       }
-      stockham<T2, L, 1, true>(v, fib, q, twf);
+      stockham<T2, L, 1, true>(v, fib, q, twf, stab);
     }
 
     if constexpr (XV == XV_MASK) {
@@ -142,7 +148,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     }
 
     if (A.do_fwd) {
-      stockham<T2, L, 1, false>(v, fib, q, twf);
+      stockham<T2, L, 1, false>(v, fib, q, twf, stab);
       if (!ROW) {
         if (have_tw) {
 #pragma unroll
@@ -160,8 +166,13 @@ __global__ void __launch_bounds__(C *(L / 16))
         const KopF32 kp = kop_f32_setup(A.k, ktab, 63 - __clzll((unsigned long long)g.N));
         const int k0 = (int)(kb + kstep * q);
         const int kst = (int)(kstep * TPF);
+        if (kp.rlen != 0) {
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = kop_f32(kp, k0 + m * kst, v[m]);
+          for (int m = 0; m < 16; ++m) v[m] = kop_f32<true>(kp, k0 + m * kst, v[m]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 16; ++m) v[m] = kop_f32<false>(kp, k0 + m * kst, v[m]);
+        }
       } else {
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -171,7 +182,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       }
     }
 
-    if (ROW && A.do_inv) stockham<T2, L, 1, true>(v, fib, q, twf);
+    if (ROW && A.do_inv) stockham<T2, L, 1, true>(v, fib, q, twf, stab);
 
     {
       const gptr<nvec2<S>> p0 = data + nb;

@@ -88,7 +88,7 @@ struct Geom {
   int32_t xcd_segs;  // column passes: workgroups sharing one 128 B line (1 = none)
   const void *tw_hi;
   const void *tw_lo;
-  const void *tw_fib;  // W_L^j, j < L
+  const void *tw_fib;  // W_L^j, j < L, then the small-stage tables (rs_kernels.hpp)
   const void *cis_tab;  // complex64 plans: exp(2 pi i j / 256), j < 256 (row pass phase)
 };
 

@@ -48,6 +48,27 @@ constexpr int kMaxColLg = 9;        // default plans: columns of up to 512 with 
 constexpr int kMaxWideColLgF32 = 13;  // wide column passes (C * L = 16384 complex64, 128 KB LDS)
 constexpr int kMaxWideColLgF64 = 12;  // (C * L = 8192 complex128)
 
+// Stockham stage sequence of a fiber of length L: the remainder radix
+// R0 (L = R0 * 16^k) at sub-length NS = 1, then radix 16 at NS = R0, 16 R0, ...
+constexpr int stage_r0(int L) {
+  int l = L;
+  while (l > 16) l /= 16;
+  return l;
+}
+// Radix-16 stages at sub-length NS <= kSmallNS read their twiddles
+// W_{16 NS}^{k r} from a table of NS rows of kTwRow entries ([k][r], one pad
+// entry per row); the tables of all such stages follow the L entries W_L^j of
+// the fiber's twiddle array, in stage order.
+constexpr int kSmallNS = 32;
+constexpr int kTwRow = 17;
+constexpr int small_tab_offset(int L, int NS) {
+  int off = 0;
+  for (int ns = stage_r0(L); ns < NS && ns < L; ns *= 16)
+    if (ns > 1 && ns <= kSmallNS) off += ns * kTwRow;
+  return off;
+}
+constexpr int small_tab_entries(int L) { return small_tab_offset(L, L); }
+
 // fibers per workgroup of a column pass of length L: 128 B row segments up to
 // L = 512 (more fibers for short L), then C * L fixed at 128 KB of LDS
 constexpr int col_fibers(int L, bool f64) {
