@@ -197,7 +197,8 @@ typedef struct rs_batch_desc {
   int32_t max_resident;    /* members resident at once, one launch per pass covering them; 0 = as many
                               as fit the 256 MiB last-level cache together (at least one), bounded by memory */
   int32_t use_graph;       /* capture each group's launch sequence in a hipGraph */
-  int32_t reserved;
+  int32_t lanes;           /* streams that groups alternate over (1..4); 0 = 2 with the default
+                              grouping, 1 otherwise */
 } rs_batch_desc;
 
 /* Run every member; out[nr_outputs * n_px] receives sum over members of
@@ -262,10 +263,10 @@ typedef struct rs_pass_stat {
 } rs_pass_stat;
 
 /* Autotuner (no counterpart in the reference): time every valid digit-group
- * split of an N-point field on a synthetic grating slice loop run over
- * `members` fields (<= 0: 1) grouped as rs_batch_run groups them by default,
- * and keep the fastest for this context and that launch width (fields per
- * launch); runs with another width use the entry tuned for the nearest one.
+ * split of an N-point field on a synthetic grating slice loop run as a batch
+ * of `members` fields (<= 0: 1) runs in rs_batch_run by default (grouping and
+ * lanes), by wall clock, and keep the fastest for this context and that batch
+ * size; runs of other batch sizes use the entry tuned for the nearest one.
  * out4 (optional) = {groups, lg top, lg middle, lg row}; ms_per_slice
  * (optional) = measured time of one slice of one field with it. */
 int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice);

@@ -567,6 +567,8 @@ int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out
     rc.n_out = d->nr_outputs;
     rc.max_resident = d->max_resident;
     rc.use_graph = d->use_graph != 0;
+    if (d->lanes < 0 || d->lanes > 4) throw rs::Error(RS_ERR_INVALID, "lanes must be 0..4");
+    rc.lanes = d->lanes;
     std::vector<double> det = rs::run_members(*ctx->ctx, rc, ms);
     std::memcpy(out, det.data(), det.size() * sizeof(double));
   });

@@ -3,6 +3,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -352,7 +353,17 @@ Context::Context(int dev) : device(dev) {
 }
 Context::~Context() {
   plans_.clear();
+  for (hipStream_t s : lanes_) (void)hipStreamDestroy(s);
   if (stream) (void)hipStreamDestroy(stream);
+}
+hipStream_t Context::lane_stream(int i) {
+  if (i == 0) return stream;
+  while ((int)lanes_.size() < i) {
+    hipStream_t s = nullptr;
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    lanes_.push_back(s);
+  }
+  return lanes_[i - 1];
 }
 const Context::Tuned *Context::find_tuned(int lgN, bool f64, int members) const {
   const Tuned *best = nullptr;
@@ -418,8 +429,6 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   hip_check(hipMemcpyAsync(dbounds.p, hb.data(), hb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream),
             "bounds upload");
   const size_t nacc = (size_t)rc.n_out * std::max<int64_t>(npx, 0);
-  DevBuf dacc(std::max<size_t>(nacc, 1) * sizeof(double));
-  hip_check(launch_fill_f64((double *)dacc.p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
 
   bool needs_snap = false;
   for (auto &m : members)
@@ -431,29 +440,55 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     group = default_group(fbytes * (needs_snap ? 2 : 1), (int64_t)members.size());
   if (rc.fields) group = (int)members.size();
   group = std::max(1, std::min(group, 65535));
-  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, std::min<int>(group, (int)members.size()));
 
-  // Field (and snapshot) buffers of one group, reused by every group: each
-  // member program starts by generating or uploading its field.  All groups
-  // are compiled first, their argument records go to the device in one copy,
-  // and the launches of all groups are enqueued back to back.
-  std::vector<DevBuf> fbuf, sbuf;
+  // Lanes: with the default grouping and more than one group, groups
+  // alternate between the context's two streams, so that a VALU-bound pass of
+  // one group runs beside a memory-bound pass of the other; the resident
+  // fields are split between the lanes.  One lane when timing per pass, under
+  // graph capture, with caller fields or an explicit group size.
+  int lanes = 1;
+  if (!rc.fields && !ctx.timing && !rc.use_graph && members.size() >= 2) {
+    if (rc.lanes > 0) lanes = rc.lanes;
+    else if (rc.max_resident <= 0) lanes = 2;
+  }
+  lanes = std::max(1, std::min({lanes, 4, (int)members.size()}));
+  if (lanes > 1 && rc.max_resident <= 0) group = std::max(1, group / lanes);
+  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535));
+  std::vector<hipStream_t> lane_st(lanes);
+  for (int l = 0; l < lanes; ++l) lane_st[l] = ctx.lane_stream(l);
+
+  // per-lane accumulators (summed in lane order on the host)
+  std::vector<DevBuf> dacc;
+  for (int l = 0; l < lanes; ++l) {
+    dacc.emplace_back(std::max<size_t>(nacc, 1) * sizeof(double));
+    hip_check(launch_fill_f64((double *)dacc.back().p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
+  }
+
+  // Field (and snapshot) buffers of one group per lane, reused by every group
+  // of the lane: each member program starts by generating or uploading its
+  // field.  All groups are compiled first, their argument records go to the
+  // device in one copy, and the launches are enqueued back to back.
+  std::vector<std::vector<DevBuf>> fbuf(lanes), sbuf(lanes);
   const size_t width = std::min(members.size(), (size_t)group);
-  if (!rc.fields)
-    for (size_t i = 0; i < width; ++i) fbuf.emplace_back(fbytes);
-  if (needs_snap)
-    for (size_t i = 0; i < width; ++i) sbuf.emplace_back(fbytes);
+  for (int l = 0; l < lanes; ++l) {
+    if (!rc.fields)
+      for (size_t i = 0; i < width; ++i) fbuf[l].emplace_back(fbytes);
+    if (needs_snap)
+      for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
+  }
   std::vector<std::unique_ptr<Exec<S>>> groups;
   size_t nfa = 0, nda = 0;
-  for (size_t g0 = 0; g0 < members.size(); g0 += group) {
+  for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {
     const size_t g1 = std::min(members.size(), g0 + group);
+    const int lane = (int)(gi % lanes);
     auto ex = std::make_unique<Exec<S>>(ctx, rc, plan);
+    ex->st = lane_st[lane];
     for (size_t i = g0; i < g1; ++i) {
       ex->mem.push_back(&members[i]);
-      ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[i - g0].p);
-      if (needs_snap) ex->snap.push_back(sbuf[i - g0].p);
+      ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);
+      if (needs_snap) ex->snap.push_back(sbuf[lane][i - g0].p);
     }
-    ex->acc = (double *)dacc.p;
+    ex->acc = (double *)dacc[lane].p;
     ex->bounds = (const int64_t *)dbounds.p;
     ex->npx = npx;
     ex->compile();
@@ -477,25 +512,45 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   if (nda)
     hip_check(hipMemcpyAsync(dda.p, hda.data(), nda * sizeof(DetArgs), hipMemcpyHostToDevice, ctx.stream),
               "det args upload");
+  // lanes start after the uploads on the context stream and join it at the end
+  std::vector<hipEvent_t> evs;
+  auto new_event = [&]() {
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    evs.push_back(e);
+    return e;
+  };
+  if (lanes > 1) {
+    hipEvent_t ready = new_event();
+    hip_check(hipEventRecord(ready, ctx.stream), "event record");
+    for (int l = 1; l < lanes; ++l) hip_check(hipStreamWaitEvent(lane_st[l], ready, 0), "stream wait");
+  }
   size_t ofa = 0, oda = 0;
   for (auto &ex : groups) {
     ex->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
     ofa += ex->fargs.size();
     oda += ex->dargs.size();
   }
+  for (int l = 1; l < lanes; ++l) {
+    hipEvent_t done = new_event();
+    hip_check(hipEventRecord(done, lane_st[l]), "event record");
+    hip_check(hipStreamWaitEvent(ctx.stream, done, 0), "stream wait");
+  }
 
-  std::vector<double> accs(nacc);
-  if (nacc)
-    hip_check(hipMemcpyAsync(accs.data(), dacc.p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+  std::vector<std::vector<double>> accs(lanes, std::vector<double>(nacc));
+  for (int l = 0; l < lanes && nacc; ++l)
+    hip_check(hipMemcpyAsync(accs[l].data(), dacc[l].p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
               "acc download");
   hip_check(hipStreamSynchronize(ctx.stream), "sync");
+  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   for (auto &ex : groups) ex->collect_timing();
 
   const std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
   std::vector<double> out(nacc);
   for (int r = 0; r < rc.n_out; ++r)
     for (int64_t k = 0; k < npx; ++k) {
-      const double v = accs[(size_t)r * npx + k];
+      double v = accs[0][(size_t)r * npx + k];
+      for (int l = 1; l < lanes; ++l) v += accs[l][(size_t)r * npx + k];
       out[(size_t)r * npx + k] = (((v * sc[2 * k]) * rc.dx) * rc.det_py) / sc[2 * k + 1];
     }
   return out;
@@ -557,33 +612,36 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members
   rc.dx = dx;
   rc.f64 = f64;
   rc.n_out = 1;
-  // fields per launch as the lockstep runner will group `members` fields;
-  // time a few groups of that width
+  // Time the runner as a batch of `members` fields runs by default (its
+  // grouping and lanes), on up to four groups' worth of fields, by wall
+  // clock around the run (lanes overlap, so kernel times would not add up).
   const int width = default_group((size_t)N * (f64 ? 16 : 8), members);
-  members = std::min(members, 4 * width);
-  rc.max_resident = width;
+  const int key = members;
+  const int run_n = std::min(members, 4 * width);
+  rc.max_resident = 0;
   const bool was = ctx.timing;
-  Context::Tuned best{lgN, f64, -1, -1, 1e300, width};
+  ctx.timing = false;
+  Context::Tuned best{lgN, f64, -1, -1, 1e300, key};
   for (auto cand : plan_candidates(lgN, f64)) {
     rc.lgR = cand.first;
     rc.lgA = cand.second;
-    std::vector<Member> ms(members, m);
-    ctx.timing = false;
+    std::vector<Member> ms(run_n, m);
     (void)run_members(ctx, rc, ms);  // warm: plan tables, code objects
-    ms.assign(members, m);
-    for (auto &st : ctx.stats) st = PassStat{};
-    ctx.timing = true;
-    (void)run_members(ctx, rc, ms);
-    double tot = 0;
-    for (auto &st : ctx.stats) tot += st.ms;
-    const double per = tot / (reps + 1) / members;
-    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, width};
+    double t_best = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      ms.assign(run_n, m);
+      hip_check(hipStreamSynchronize(ctx.stream), "sync");
+      const auto t0 = std::chrono::steady_clock::now();
+      (void)run_members(ctx, rc, ms);
+      t_best = std::min(t_best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    const double per = t_best / (reps + 1) / run_n;
+    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, key};
   }
   ctx.timing = was;
-  for (auto &st : ctx.stats) st = PassStat{};
   bool replaced = false;
   for (auto &t : ctx.tuned)
-    if (t.lgN == lgN && t.f64 == f64 && (t.members == width || t.members == 0)) {
+    if (t.lgN == lgN && t.f64 == f64 && (t.members == key || t.members == 0)) {
       t = best;
       replaced = true;
       break;

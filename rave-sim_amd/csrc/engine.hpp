@@ -120,6 +120,7 @@ struct RunConfig {
   int n_out = 1;             // accumulator rows
   int max_resident = 0;      // members per group; 0 = as many as fit the last-level cache together
   bool use_graph = false;
+  int lanes = 0;             // streams groups alternate over; 0 = 2 with the default grouping
   int lgR = -1;              // row length override (autotuner), -1 = tuned or default
   int lgA = -1;              // top column length override
   void *const *fields = nullptr;  // caller-owned field buffers (one per member), else allocated
@@ -147,15 +148,18 @@ class Context {
   explicit Context(int device);
   ~Context();
   int device;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // all work of the context's calls is ordered on it
+  // lane i of the lockstep runner (0 = `stream`; others created on first use
+  // and joined into `stream` at the end of each run)
+  hipStream_t lane_stream(int i);
   bool timing = false;
   PassStat stats[PC_COUNT];
   // plan for (N, precision); explicit lgR/lgA, else the split tuned for the
-  // nearest number of fields per launch, else the default split
+  // nearest batch size (fields per run), else the default split
   Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1, int members = 1);
   void sync();
-  // autotuner results: (lgN, f64, fields per launch) -> (lgR, lgA, ms per
-  // slice and field); members == 0 marks a fixed split for every batch size
+  // autotuner results: (lgN, f64, fields per run) -> (lgR, lgA, ms per slice
+  // and field); members == 0 marks a fixed split for every batch size
   struct Tuned {
     int lgN;
     bool f64;
@@ -167,14 +171,15 @@ class Context {
   const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
 
  private:
+  std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;
 };
 
 // Time every candidate split of N on a synthetic grating slice loop (mask +
-// propagation) run over `members` fields grouped as run_members groups them
-// by default (one launch per pass covers a group), and record the fastest in
-// ctx.tuned under that launch width.  Returns it.
+// propagation) run as a batch of `members` fields runs by default (grouping
+// and lanes of run_members), by wall clock, and record the fastest in
+// ctx.tuned for that batch size.  Returns it.
 Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3, int members = 1);
 
 // device buffer RAII

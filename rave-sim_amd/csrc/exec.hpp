@@ -158,8 +158,9 @@ template <typename S> struct Exec {
   double *acc = nullptr;
   const int64_t *bounds = nullptr;
   int64_t npx = 0;
+  hipStream_t st;  // stream the launches go to (the context stream by default)
 
-  Exec(Context &c, const RunConfig &r, Plan &p) : ctx(c), rc(r), plan(p) {}
+  Exec(Context &c, const RunConfig &r, Plan &p) : ctx(c), rc(r), plan(p), st(c.stream) {}
 
   size_t push_args(const std::vector<FieldArgs> &v) {
     const size_t off = fargs.size();
@@ -430,7 +431,6 @@ template <typename S> struct Exec {
 
   // Upload this program's argument records, enqueue it and wait for it.
   void execute() {
-    hipStream_t st = ctx.stream;
     DevBuf dfa(std::max<size_t>(fargs.size(), 1) * sizeof(FieldArgs));
     DevBuf dda(std::max<size_t>(dargs.size(), 1) * sizeof(DetArgs));
     if (!fargs.empty())
@@ -448,7 +448,6 @@ template <typename S> struct Exec {
   // on the device (fa[i] = fargs[i], da[i] = dargs[i]); returns without
   // waiting unless a graph or a host download needs it.
   void enqueue(const FieldArgs *fa, const DetArgs *da) {
-    hipStream_t st = ctx.stream;
     const size_t esz = 2 * sizeof(S);
 
     // graph capture covers the run of launches between host interactions

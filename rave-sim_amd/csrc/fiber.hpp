@@ -103,9 +103,9 @@ __global__ void __launch_bounds__(C *(L / 16))
     const int64_t col = c0 + c;
     const int64_t rr = tile * C + c;
 
-    // inter-group twiddles: top fibers up to 256 keep them live from the
-    // inverse to the forward DFT; otherwise they are recomputed (registers)
-    constexpr bool KEEP_TW = !ROW && L <= 256 && XV != XV_NONE;
+    // inter-group twiddles: top passes keep them live from the inverse to
+    // the forward DFT; middle passes (one direction) apply them on the fly
+    constexpr bool KEEP_TW = !ROW && XV != XV_NONE;
     T2 tw[16];
     bool have_tw = false;
     if (!ROW && A.do_inv) {

@@ -238,3 +238,32 @@ def test_autotune_per_batch_size():
     default = run_batch(d, N, dx, ms, **kw)
     d.close()
     assert rel_l2(tuned, default) < 1e-5
+
+
+# every row and column fiber length of both precisions through a forced
+# two-group split (row 2^lgR, column 2^lgA), N <= 2^21: each Stockham stage
+# shape (remainder radix 2/4/8/16, small-stage twiddle tables, powers from
+# W_L) runs once against the oracle
+FIBER_CASES = ([(np.complex64, r, a) for r in range(6, 14) for a in (4, 7)] +
+               [(np.complex64, 6, a) for a in range(10, 14)] +
+               [(np.complex128, r, a) for r in range(6, 13) for a in (5, 8)] +
+               [(np.complex128, 6, a) for a in range(10, 13)])
+
+
+@pytest.mark.parametrize("dtype,lgR,lgA", FIBER_CASES)
+def test_every_fiber_length(ctx, dtype, lgR, lgA):
+    N = 2 ** (lgR + lgA)
+    ctx.set_split(N, dtype, lgR, lgA)
+    dx, wl, dz = 2e-9, 2.7e-11, 2.3e-3
+    u = rand_field(N, 1000 + lgR * 16 + lgA)
+    fc = 0.35 / dx
+    import ravesim
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.upload(u)
+    f.propagate(wl, dz, fc)
+    got = f.download()
+    f.close()
+    ref = bw.Fields(u.astype(np.complex128))
+    bw.propagate(ref, dx, wl, dz, N, fc)
+    tol = 1e-10 if dtype == np.complex128 else 2e-6
+    assert rel_l2(got, ref.u) < tol
