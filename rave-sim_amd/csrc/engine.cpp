@@ -496,6 +496,15 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     nda += ex->dargs.size();
     groups.push_back(std::move(ex));
   }
+  // Grating decisions shared by several passes: computed once, read as bits
+  // (N/8 bytes per mask).  Only when the resident fields overflow the
+  // last-level cache anyway: when they fit, the extra stream evicts them and
+  // costs more than the decisions it saves.
+  MaskCache<S> masks;
+  const size_t resident = (size_t)lanes * width * fbytes * (needs_snap ? 2 : 1);
+  if (rc.mode == 0 && resident > kMallBytes)
+    for (auto &ex : groups) masks.collect(*ex);
+  masks.fill(rc.N, ctx.stream);
   std::vector<FieldArgs> hfa;
   std::vector<DetArgs> hda;
   hfa.reserve(nfa);

@@ -562,5 +562,58 @@ template <typename S> struct Exec {
 };
 
 
+// Grating masks shared by several top column passes of one run (the slices of
+// a grating with constant duty cycle, the members of a batch): their
+// decisions are computed once per run by mask_fill_kernel and read as bits by
+// the passes.  Masks used by a single pass keep the inline test.
+template <typename S> struct MaskCache {
+  struct Entry {
+    XOp key;
+    int lgL;
+    Geom g;
+    std::vector<FieldArgs *> recs;
+    DevBuf bits;
+  };
+  std::vector<Entry> entries;
+
+  static bool same(const XOp &a, const XOp &b) {
+    const double ka[8] = {a.m_a, a.m_b, a.m_guard, a.dc, a.pitch, a.half_dc, a.x0, a.xstep};
+    const double kb[8] = {b.m_a, b.m_b, b.m_guard, b.dc, b.pitch, b.half_dc, b.x0, b.xstep};
+    return std::memcmp(ka, kb, sizeof ka) == 0;
+  }
+
+  void collect(Exec<S> &ex) {
+    for (const Launch &l : ex.launches) {
+      if (l.kind != LKind::Fiber || l.row || l.xv != XV_MASK) continue;
+      for (int i = 0; i < l.nf; ++i) {
+        FieldArgs &a = ex.fargs[l.arg_off + i];
+        if (a.x.kind != XOP_MASK || !a.x.single_chunk) continue;
+        Entry *e = nullptr;
+        for (auto &c : entries)
+          if (c.lgL == l.lgL && same(c.key, a.x)) {
+            e = &c;
+            break;
+          }
+        if (!e) {
+          entries.push_back(Entry{a.x, l.lgL, l.geom, {}, DevBuf()});
+          e = &entries.back();
+        }
+        e->recs.push_back(&a);
+      }
+    }
+  }
+
+  // fill the masks used more than once on stream st and point their records
+  // at them (before the records are uploaded)
+  void fill(int64_t N, hipStream_t st) {
+    for (auto &e : entries) {
+      if (e.recs.size() < 2) continue;
+      e.bits = DevBuf((size_t)N / 8);
+      hip_check(launch_mask_fill<S>(e.lgL, e.key, e.g, e.bits.p, st), "mask fill");
+      for (FieldArgs *r : e.recs) r->x.mbits = e.bits.p;
+    }
+  }
+};
+
 }  // namespace exec
 }  // namespace rs

@@ -15,6 +15,47 @@
 
 namespace rs {
 
+// Element map of a fiber-pass tile: thread t of tile `tile` holds samples
+// nb + lane + m * mst (m < 16) of fiber c (column c0 + c for column passes),
+// fiber positions q + m * L/16.  nb and mst are uniform over the workgroup,
+// lane * sizeof(complex) < 2^31 for N <= 2^30.
+template <int L, int C, bool ROW> struct TileMap {
+  int c, q;
+  int64_t nb, c0, mst;
+  uint32_t lane;
+  __device__ __forceinline__ TileMap(const Geom &g, int64_t tile, int t) {
+    constexpr int TPF = L / 16;
+    c = ROW ? t / TPF : t % C;
+    q = ROW ? t % TPF : t / C;
+    if (ROW) {
+      nb = tile * C * L;
+      c0 = 0;
+      mst = TPF;
+      lane = (uint32_t)(c * L + q);
+      return;
+    }
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; when a
+    // row segment of C columns is shorter than a 128 B line, the `segs`
+    // tiles that share one line are given ids on the same XCD so that its L2
+    // merges their partial-line reads and writes
+    const int64_t ntiles = g.N / ((int64_t)L * C);
+    int64_t bid = tile;
+    if (g.xcd_segs > 1) {
+      const int64_t G = 8 * (int64_t)g.xcd_segs;
+      if (bid < (ntiles / G) * G) {
+        const int64_t r = bid % G;
+        bid = (bid - r) + (r % 8) * g.xcd_segs + r / 8;
+      }
+    }
+    const int64_t cpb = g.S / C;
+    const int64_t hi = bid / cpb;
+    c0 = (bid % cpb) * C;
+    nb = hi * (int64_t)L * g.S + c0;
+    mst = (int64_t)TPF * g.S;
+    lane = (uint32_t)(c + (int64_t)q * g.S);
+  }
+};
+
 template <typename S, int L, int C, bool ROW, int XV>
 __global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
@@ -45,41 +86,13 @@ __global__ void __launch_bounds__(C *(L / 16))
       __syncthreads();
     }
   }
-  const int c = ROW ? t / TPF : t % C;
-  const int q = ROW ? t % TPF : t / C;
-  T2 *fib = lds + c * FIB;
-
-  // Sample index of v[m] in the workgroup's tile: nb + lane + m * mst, with
-  // nb and mst uniform over the workgroup and lane * sizeof(T2) < 2^31 for
-  // N <= 2^30.
-  const int64_t ntiles = g.N / ((int64_t)L * C);
-  const int64_t mst = ROW ? (int64_t)TPF : (int64_t)TPF * g.S;
-  const uint32_t lane = ROW ? (uint32_t)(c * L + q) : (uint32_t)(c + (int64_t)q * g.S);
+  const int64_t tile = blockIdx.x;
+  const TileMap<L, C, ROW> tm(g, tile, t);
+  const int c = tm.c, q = tm.q;
+  const int64_t mst = tm.mst;
+  const uint32_t lane = tm.lane;
   const uint32_t lb = lane * (uint32_t)sizeof(T2);
-  const int64_t cpb = ROW ? 1 : g.S / C;
-  // first sample (nb) and first column of a tile
-  auto tile_base = [&](int64_t tile, int64_t &nb, int64_t &c0) {
-    if (ROW) {
-      nb = tile * C * L;
-      c0 = 0;
-      return;
-    }
-    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; when a
-    // row segment of C columns is shorter than a 128 B line, the `segs`
-    // tiles that share one line are given ids on the same XCD so that its L2
-    // merges their partial-line reads and writes
-    int64_t bid = tile;
-    if (g.xcd_segs > 1) {
-      const int64_t G = 8 * (int64_t)g.xcd_segs;
-      if (bid < (ntiles / G) * G) {
-        const int64_t r = bid % G;
-        bid = (bid - r) + (r % 8) * g.xcd_segs + r / 8;
-      }
-    }
-    const int64_t hi = bid / cpb;
-    c0 = (bid % cpb) * C;
-    nb = hi * (int64_t)L * g.S + c0;
-  };
+  T2 *fib = lds + c * FIB;
 
   const gptr<nvec2<S>> data = global_ptr<S>(A.data);
   const bool xop = XV != XV_NONE && g.top && A.x.kind != XOP_NONE;
@@ -87,9 +100,12 @@ __global__ void __launch_bounds__(C *(L / 16))
   if constexpr (XV == XV_MASK) gen = xop && A.x.from_ones;
   if constexpr (XV == XV_GEN) gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones);
 
-  const int64_t tile = blockIdx.x;
-  int64_t nb, c0;
-  tile_base(tile, nb, c0);
+  const int64_t nb = tm.nb, c0 = tm.c0;
+  // cached grating decisions of this thread, loaded with the field (their
+  // first use comes after the inverse DFT's barriers)
+  uint32_t mword = 0;
+  if constexpr (XV == XV_MASK && !ROW)
+    if (xop && A.x.kind == XOP_MASK && A.x.mbits) mword = ((const uint16_t *)A.x.mbits)[tile * (C * TPF) + t];
   T2 v[16];
   if (!gen) {
     const gptr<nvec2<S>> p0 = data + nb;
@@ -126,7 +142,14 @@ __global__ void __launch_bounds__(C *(L / 16))
         T2 fa, fb;
         fa.x = (S)x.fa[0]; fa.y = (S)x.fa[1];
         fb.x = (S)x.fb[0]; fb.y = (S)x.fb[1];
-        if (x.kind == XOP_MASK) {
+        if (x.kind == XOP_MASK && !ROW && x.mbits) {
+          // decisions cached for this tile shape (mask_fill_kernel)
+#pragma unroll
+          for (int m = 0; m < 16; ++m) {
+            const T2 f = ((mword >> m) & 1u) ? fa : fb;
+            v[m] = gen ? f : cmul(v[m], f);
+          }
+        } else if (x.kind == XOP_MASK) {
           const double omg = 1.0 - x.m_guard;
           const int32_t n0 = (int32_t)(nb + lane);
           const int32_t ms = (int32_t)mst;
@@ -190,6 +213,47 @@ __global__ void __launch_bounds__(C *(L / 16))
       for (int m = 0; m < 16; ++m) gstore<S>(p0 + m * mst, lb, v[m]);
     }
 
+  }
+}
+
+// Grating decisions of a top column pass of fiber length L, in the order the
+// pass visits its samples (TileMap): bit m of word [tile * threads + t].
+template <int L, int C>
+__global__ void __launch_bounds__(C *(L / 16)) mask_fill_kernel(XOp x, Geom g, uint16_t *__restrict__ bits) {
+  const TileMap<L, C, false> tm(g, blockIdx.x, threadIdx.x);
+  const double omg = 1.0 - x.m_guard;
+  const int32_t n0 = (int32_t)(tm.nb + tm.lane);
+  const int32_t ms = (int32_t)tm.mst;
+  uint32_t w = 0;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) w |= (mask_in(x, omg, n0 + m * ms) ? 1u : 0u) << m;
+  bits[(int64_t)blockIdx.x * (C * (L / 16)) + threadIdx.x] = (uint16_t)w;
+}
+
+template <typename S, int L>
+static hipError_t launch_mask_fill_l(const XOp &x, const Geom &g, void *bits, hipStream_t st) {
+  constexpr int C = col_fibers(L, sizeof(S) == 8);
+  if (g.S % C != 0) return hipErrorInvalidValue;
+  const int64_t nblk = g.N / ((int64_t)L * C);
+  hipLaunchKernelGGL((mask_fill_kernel<L, C>), dim3((unsigned)nblk), dim3(C * (L / 16)), 0, st, x, g,
+                     (uint16_t *)bits);
+  return hipGetLastError();
+}
+
+template <typename S>
+hipError_t launch_mask_fill_t(int lgL, const XOp &x, const Geom &g, void *bits, hipStream_t st) {
+  switch (lgL) {
+    case 4: return launch_mask_fill_l<S, 16>(x, g, bits, st);
+    case 5: return launch_mask_fill_l<S, 32>(x, g, bits, st);
+    case 6: return launch_mask_fill_l<S, 64>(x, g, bits, st);
+    case 7: return launch_mask_fill_l<S, 128>(x, g, bits, st);
+    case 8: return launch_mask_fill_l<S, 256>(x, g, bits, st);
+    case 9: return launch_mask_fill_l<S, 512>(x, g, bits, st);
+    case 10: return launch_mask_fill_l<S, 1024>(x, g, bits, st);
+    case 11: return launch_mask_fill_l<S, 2048>(x, g, bits, st);
+    case 12: return launch_mask_fill_l<S, 4096>(x, g, bits, st);
+    case 13: return launch_mask_fill_l<S, 8192>(x, g, bits, st);
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -1,6 +1,8 @@
 // gfx950 kernels of the split-step path other than the fiber passes
 // (fiber.hpp, instantiated by fiber_inst.hip):
 //
+//  mask_fill_kernel (fiber.hpp) -- grating decisions of a top column pass,
+//                  cached as bits for the passes of a run that share them.
 //  xop_kernel   -- x-space multiply / generation as its own pass (plain form).
 //  kop_kernel   -- spectrum multiply as its own pass (plain form).
 //  detect_kernel-- per-pixel FP64 sums of |u|^2 over host-computed sample
@@ -15,6 +17,7 @@
 #include <cstdio>
 
 #include "device_ops.hpp"
+#include "fiber.hpp"
 #include "rs_kernels.hpp"
 
 namespace rs {
@@ -135,6 +138,11 @@ hipError_t launch_fiber_pass(int lgL, bool row, int xv, const FieldArgs *args, i
 }
 
 template <typename S>
+hipError_t launch_mask_fill(int lgL, const XOp &x, const Geom &g, void *bits, hipStream_t st) {
+  return launch_mask_fill_t<S>(lgL, x, g, bits, st);
+}
+
+template <typename S>
 hipError_t launch_xop(const FieldArgs *args, int nf, int64_t N, hipStream_t st) {
   hipLaunchKernelGGL(xop_kernel<S>, dim3(grid_stride_blocks(N), nf), dim3(256), 0, st, args, N);
   return hipGetLastError();
@@ -169,6 +177,8 @@ hipError_t launch_copy(void *dst, const void *src, int64_t n, hipStream_t st) {
 
 template hipError_t launch_fiber_pass<float>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t);
 template hipError_t launch_fiber_pass<double>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t);
+template hipError_t launch_mask_fill<float>(int, const XOp &, const Geom &, void *, hipStream_t);
+template hipError_t launch_mask_fill<double>(int, const XOp &, const Geom &, void *, hipStream_t);
 template hipError_t launch_xop<float>(const FieldArgs *, int, int64_t, hipStream_t);
 template hipError_t launch_xop<double>(const FieldArgs *, int, int64_t, hipStream_t);
 template hipError_t launch_kop<float>(const FieldArgs *, int, const Geom &, hipStream_t);

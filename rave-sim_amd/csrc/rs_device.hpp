@@ -43,6 +43,9 @@ struct XOp {
   // t ~ fma(n, m_a, m_b) is within m_guard of big-wave's x/pitch + dc/2, and
   // samples whose fraction lies within m_guard of an edge use the exact path
   double m_a, m_b, m_guard;
+  // optional cached decisions of a top column pass (host-filled per run):
+  // bit m of word [tile * threads + thread] is the decision for v[m]
+  const void *mbits;
 };
 
 // compile-time x-space variants of a fiber pass: none (middle / row passes),

@@ -33,6 +33,10 @@ inline int xop_variant(const FieldArgs *a, int nf, bool top) {
   return v;
 }
 template <typename S> hipError_t launch_xop(const FieldArgs *args, int nf, int64_t N, hipStream_t st);
+// grating decisions of a top column pass of fiber length 2^lgL for the mask x
+// (single chunk, finalized), as bits in the pass's sample order: N/8 bytes
+template <typename S>
+hipError_t launch_mask_fill(int lgL, const XOp &x, const Geom &g, void *bits, hipStream_t st);
 template <typename S> hipError_t launch_kop(const FieldArgs *args, int nf, const Geom &g, hipStream_t st);
 template <typename S>
 hipError_t launch_detect(const DetArgs *args, int nf, const int64_t *bounds, int64_t npx, hipStream_t st);

@@ -99,6 +99,33 @@ def test_grating_mask_bit_exact(ctx, N, chunk, xpos, dtype):
     assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("N,xpos,dtype", [(2 ** 14, 0.0, np.complex64), (2 ** 20, -1.05e-6, np.complex128),
+                                           (2 ** 26, 3.3e-7, np.complex64), (2 ** 25, 0.0, np.complex128)])
+def test_grating_mask_cached_bit_exact(ctx, N, xpos, dtype):
+    """Slices of a constant-duty-cycle grating share one mask.  Fields larger
+    than the 256 MiB last-level cache (the last two cases) get it computed once
+    per run and read back as bits in every slice's top pass; smaller ones
+    decide inline.  Both stay big-wave's decisions at exact ties (pitch/dx =
+    2100)."""
+    import ravesim
+    dx, wl = 2e-9, 2.7e-11
+    t = 1e-12
+    beta = math.log(2) * wl / (2 * math.pi * t)  # each slice halves the amplitude in A
+    g = ravesim.Grating(pitch=4.2e-6, dc=(0.41, 0.41), z_start=0.0, thickness=4 * t, nr_steps=4,
+                        x_positions=[xpos], db_a=complex(0, beta))
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.upload(np.ones(N))
+    f.apply_grating(wl, g, phase_step=0)
+    a = np.abs(f.download())
+    f.close()
+    og = bw.Grating(pitch=4.2e-6, dc=(0.41, 0.41), z_start=0.0, thickness=t, nr_steps=1,
+                    x_positions=np.array([xpos]), substrate_thickness=0.0)
+    exp = bw.grating_mask(dx, -(N // 2), N, og, 0.0, 0)
+    got = a < 0.25  # 1/16 inside A, ~1 outside
+    assert np.array_equal(got, exp)
+    assert np.all(np.abs(a[exp] - 1.0 / 16.0) < 1e-3) and np.all(np.abs(a[~exp] - 1.0) < 1e-3)
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_point_source_matches_oracle(ctx, dtype):
     import ravesim
