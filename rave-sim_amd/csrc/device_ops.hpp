@@ -352,6 +352,45 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
       re = 1.0; im = 0.0;
       mul = false;
       break;
+    case XOP_SAMPLE: {
+      // delta + i beta by np.interp(x, rx, db, left=0, right=0) per
+      // component, then exp(2 pi i dz / wl * db) (big-wave/optical_element.py
+      // :449-470); intervals between equal pixels and exact knots use the
+      // host-computed pixel factor
+      const double x = ((double)n - (double)N / 2.0) * op.dx;
+      const double *rx = op.s_rx;
+      const int64_t len = op.s_len;
+      re = 1.0;
+      im = 0.0;
+      if (x < rx[0] || x > rx[len - 1]) break;  // db = 0
+      if (x == rx[len - 1]) {
+        re = op.s_f[2 * (len - 1)];
+        im = op.s_f[2 * (len - 1) + 1];
+        break;
+      }
+      int64_t j = (int64_t)floor((x - rx[0]) * op.s_inv_px);
+      if (j < 0) j = 0;
+      if (j > len - 2) j = len - 2;
+      while (j > 0 && rx[j] > x) --j;
+      while (j < len - 2 && rx[j + 1] <= x) ++j;
+      const double *db = op.s_db;
+      if (rx[j] == x || (db[2 * j] == db[2 * j + 2] && db[2 * j + 1] == db[2 * j + 3])) {
+        re = op.s_f[2 * j];
+        im = op.s_f[2 * j + 1];
+        break;
+      }
+      const double span = rx[j + 1] - rx[j];
+      const double dr = ((db[2 * j + 2] - db[2 * j]) / span) * (x - rx[j]) + db[2 * j];
+      const double di = ((db[2 * j + 3] - db[2 * j + 1]) / span) * (x - rx[j]) + db[2 * j + 1];
+      const double er = -(op.s_a * di) + 0.0 * dr;
+      const double ei = op.s_a * dr + 0.0 * di;
+      const double e = exp(er);
+      double sn, cs;
+      sincos(ei, &sn, &cs);
+      re = e * cs;
+      im = e * sn;
+      break;
+    }
     default:
       return v;
   }

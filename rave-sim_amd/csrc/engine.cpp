@@ -476,6 +476,36 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     if (needs_snap)
       for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
   }
+  // sample elements: device copies of the members' pixel tables for this run,
+  // pointed at by their SAMPLE masks
+  std::vector<DevBuf> sample_bufs;
+  auto upload_d = [&](const std::vector<double> &h) -> const double * {
+    sample_bufs.emplace_back(std::max<size_t>(h.size(), 1) * sizeof(double));
+    hip_check(hipMemcpyAsync(sample_bufs.back().p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                             ctx.stream),
+              "sample upload");
+    return (const double *)sample_bufs.back().p;
+  };
+  for (auto &m : members) {
+    if (m.samples.empty()) continue;
+    std::vector<const double *> rx, db, f;
+    for (auto &sr : m.samples) {
+      rx.push_back(upload_d(sr.rx));
+      db.push_back(upload_d(sr.db));
+      f.push_back(upload_d(sr.f));
+    }
+    for (auto &op : m.ops) {
+      if (op.sample < 0) continue;
+      const SampleRows &sr = m.samples.at((size_t)op.sample);
+      op.x.s_rx = rx[(size_t)op.sample];
+      op.x.s_db = db[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
+      op.x.s_f = f[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
+      op.x.s_len = sr.len;
+    }
+  }
+  // the member's host arrays need not outlive the copies
+  if (!sample_bufs.empty()) hip_check(hipStreamSynchronize(ctx.stream), "sample upload sync");
+
   std::vector<std::unique_ptr<Exec<S>>> groups;
   size_t nfa = 0, nda = 0;
   for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {

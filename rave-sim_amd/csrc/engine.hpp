@@ -96,10 +96,20 @@ enum class OpKind { Source, Upload, Mask, Prop, Detect, Snapshot, Restore, SaveU
 
 struct Op {
   OpKind kind;
-  XOp x{};          // Source (POINT / ONES) and Mask (MASK / SCALAR)
+  XOp x{};          // Source (POINT / ONES) and Mask (MASK / SCALAR / SAMPLE)
   double dz = 0;    // Prop
   double fc = 0;    // Prop cutoff frequency (big-wave semantics)
   int row = 0;      // Detect: output row; SaveU: phase step
+  int sample = -1;  // SAMPLE masks: index into Member::samples and the grid row
+  int sample_row = 0;
+};
+
+// Host arrays of one sample element at one phase step: pixel positions, and
+// per grid row the pixels' delta + i beta and exp(i a db) (interleaved)
+struct SampleRows {
+  std::vector<double> rx;
+  std::vector<double> db, f;  // rows x (2 len)
+  int64_t len = 0;
 };
 
 struct Member {
@@ -107,6 +117,7 @@ struct Member {
   double weight = 1.0;
   int out_base = 0;             // accumulator row of Detect(row = 0)
   std::vector<Op> ops;
+  std::vector<SampleRows> samples;  // device copies are made per run
   const void *upload = nullptr; // host data for the Upload op (natural order)
   std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
 };

@@ -89,7 +89,7 @@ inline XOp scalar_xop(double re, double im) {
   return x;
 }
 
-// x-space op composition: first `a`, then `b` (b is a MASK or SCALAR)
+// x-space op composition: first `a`, then `b` (b is a MASK, SAMPLE or SCALAR)
 inline XOp compose(const XOp &a, const XOp &b) {
   if (a.kind == XOP_NONE) return b;
   XOp r = a;
@@ -103,7 +103,7 @@ inline XOp compose(const XOp &a, const XOp &b) {
     r.post[1] = p1;
     return r;
   }
-  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) && b.kind == XOP_MASK) {
+  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) && (b.kind == XOP_MASK || b.kind == XOP_SAMPLE)) {
     XOp m = b;
     // value of `a` at every sample: fa * post for a scalar, post for ones
     const double s0 = a.kind == XOP_ONES ? a.post[0] : a.fa[0] * a.post[0] - a.fa[1] * a.post[1];

@@ -21,6 +21,7 @@ enum XopKind : int32_t {
   XOP_SCALAR = 2,  // u *= fa
   XOP_POINT = 3,   // u = exp(i kphase r) / sqrt(r)   (generated, input not read)
   XOP_ONES = 4,    // u = 1 (plane wave, generated)
+  XOP_SAMPLE = 5,  // sample row: u *= exp(i a interp(x; rx, db)) (big-wave Sample.apply)
 };
 
 // x-space (natural order) elementwise operation.  Every kind is followed by a
@@ -46,6 +47,14 @@ struct XOp {
   // optional cached decisions of a top column pass (host-filled per run):
   // bit m of word [tile * threads + thread] is the decision for v[m]
   const void *mbits;
+  // sample row (big-wave/optical_element.py:432-476): pixel positions rx[k],
+  // delta + i beta per pixel (interleaved), the factor exp(i a db) per pixel
+  // (interleaved), a = 2 pi pixel_size_z / wl; x = (n - N/2) dx
+  const double *s_rx;
+  const double *s_db;
+  const double *s_f;
+  int64_t s_len;
+  double s_inv_px, s_a;
 };
 
 // compile-time x-space variants of a fiber pass: none (middle / row passes),

@@ -153,6 +153,8 @@ int64_t Server::submit(ServeRequest req) {
         c.px != c0.px || c.z_detector != c0.z_detector)
       throw Error(1, "all members of a request share N, precision, grid and detector");
     if (c.source.kind == SourceCfg::Vector) throw Error(6, "served requests use point or plane-wave sources");
+    for (const auto &g : c.elements)
+      if (g.sample) throw Error(6, "served requests hold gratings only");
     Member tmp;
     const int ns = build_member(c, tmp, 0, false);  // validates the configuration
     if (nsteps0 < 0) nsteps0 = ns;

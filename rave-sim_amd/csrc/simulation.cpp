@@ -1,8 +1,10 @@
 #include "simulation.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
+#include "npy.hpp"
 #include "yaml_lite.hpp"
 
 #pragma STDC FP_CONTRACT OFF
@@ -103,8 +105,30 @@ SimConfig parse_simdir(const std::string &sim_dir, int source_idx) {
   for (size_t i = 0; i < els.size(); ++i) {
     const YNode &e = els[i];
     const std::string type = e["type"].as_string("type");
+    if (type == "sample") {
+      // big-wave/config.py:118-131: grid from an int8 .npy next to config.yaml
+      GratingCfg g;
+      g.sample = true;
+      g.z_start = get_f(e, "z_start");
+      g.pixel_size_x = get_f(e, "pixel_size_x");
+      g.pixel_size_z = get_f(e, "pixel_size_z");
+      g.x_positions.clear();
+      const YNode &xp = e["x_positions"];
+      for (size_t k = 0; k < xp.size(); ++k) g.x_positions.push_back(xp[k].as_double("x_positions"));
+      const NpyArray a = npy_load(sim_dir + "/" + e["grid_path"].as_string("grid_path"));
+      if (a.descr != "|i1" && a.descr != "<i1") throw Error(2, "sample grid must be int8");
+      if (a.shape.size() != 2) throw Error(2, "sample grid must be two-dimensional");
+      g.rows = a.shape[0];
+      g.cols = a.shape[1];
+      g.grid.assign((const int8_t *)a.data.data(), (const int8_t *)a.data.data() + a.count());
+      const YNode &mats = e["materials"];
+      g.db_list.assign(1, {0, 0});
+      for (size_t k = 0; k < mats.size(); ++k) g.db_list.push_back(lookup(table, mats[k], "materials"));
+      c.elements.push_back(g);
+      continue;
+    }
     if (type != "grating")
-      throw Error(6, "optical element type '" + type + "' is not supported by this build (grating only)");
+      throw Error(6, "optical element type '" + type + "' is not supported by this build (grating, sample)");
     GratingCfg g;
     g.pitch = get_f(e, "pitch");
     g.dc0 = e["dc"][0].as_double("dc.0");
@@ -140,9 +164,22 @@ void check_config(const SimConfig &c) {
   double cur = c.source.z;
   size_t nsteps = 0;
   for (const auto &g : c.elements) {
-    if (!(g.thickness > 0)) throw Error(2, "grating thickness must be positive");
+    if (g.sample) {
+      // big-wave Sample.check_valid (optical_element.py:410-419)
+      if (g.rows <= 0 || g.cols <= 0) throw Error(2, "sample grid must not be empty");
+      if (!(g.z_start > 0)) throw Error(2, "sample z_start must be positive");
+      if (!(g.pixel_size_x > 0) || !(g.pixel_size_z > 0)) throw Error(2, "sample pixel sizes must be positive");
+      int maxidx = 0;
+      for (int8_t v : g.grid) {
+        if (v < 0) throw Error(2, "sample grid indices must be >= 0");
+        maxidx = std::max(maxidx, (int)v);
+      }
+      if (maxidx >= (int)g.db_list.size()) throw Error(2, "sample grid indexes more materials than given");
+    } else if (!(g.thickness > 0)) {
+      throw Error(2, "grating thickness must be positive");
+    }
     if (g.substrate_thickness < 0) throw Error(2, "substrate thickness must be >= 0");
-    if (g.nr_steps < 1) throw Error(2, "nr_steps must be >= 1");
+    if (!g.sample && g.nr_steps < 1) throw Error(2, "nr_steps must be >= 1");
     if (g.x_positions.empty()) throw Error(2, "x_positions must not be empty");
     if (g.x_positions.size() > 1) {
       if (nsteps && nsteps != g.x_positions.size())
@@ -219,6 +256,48 @@ void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, s
   }
 }
 
+void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m) {
+  // row_x = np.arange(rowlen) * px + x_positions[step] - rowlen * px * 0.5
+  SampleRows sr;
+  sr.len = g.cols;
+  sr.rx.resize((size_t)g.cols);
+  const double xpos = g.x_positions[(size_t)step];
+  const double half = (double)g.cols * g.pixel_size_x * 0.5;
+  for (int64_t k = 0; k < g.cols; ++k) sr.rx[(size_t)k] = ((double)k * g.pixel_size_x + xpos) - half;
+  sr.db.resize((size_t)(2 * g.rows * g.cols));
+  sr.f.resize((size_t)(2 * g.rows * g.cols));
+  for (int64_t r = 0; r < g.rows; ++r)
+    for (int64_t k = 0; k < g.cols; ++k) {
+      const std::complex<double> db = g.db_list[(size_t)g.grid[(size_t)(r * g.cols + k)]];
+      const size_t o = (size_t)(2 * (r * g.cols + k));
+      sr.db[o] = db.real();
+      sr.db[o + 1] = db.imag();
+      double f[2];
+      material_factor(db.real(), db.imag(), g.pixel_size_z, c.wl, f);
+      sr.f[o] = f[0];
+      sr.f[o + 1] = f[1];
+    }
+  const int si = (int)m.samples.size();
+  m.samples.push_back(std::move(sr));
+  const double a = (2.0 * M_PI * g.pixel_size_z) / c.wl;  // as material_factor forms it
+  for (int64_t r = 0; r < g.rows; ++r) {
+    Op mk{OpKind::Mask};
+    mk.x.kind = XOP_SAMPLE;
+    mk.x.post[0] = 1.0;
+    mk.x.post[1] = 0.0;
+    mk.x.dx = c.dx;
+    mk.x.s_inv_px = 1.0 / g.pixel_size_x;
+    mk.x.s_a = a;
+    mk.sample = si;
+    mk.sample_row = (int)r;
+    m.ops.push_back(mk);
+    Op p{OpKind::Prop};
+    p.dz = g.pixel_size_z;
+    p.fc = fc;
+    m.ops.push_back(p);
+  }
+}
+
 namespace {
 
 void grid_density_check(double dz, double xs, int64_t N, double dx, double wl) {
@@ -239,6 +318,7 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
   m.out_base = out_base;
   std::vector<Op> &ops = m.ops;
   ops.clear();
+  m.samples.clear();
   std::vector<double> fcs;
   for (double a : c.cutoff_angles) fcs.push_back(std::sin(a) / c.wl);
   const double tol = 1e-8;
@@ -298,7 +378,8 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
       ops.push_back(Op{OpKind::Snapshot});
     }
     fc = fcs[i + 1];
-    grating_ops(c, g, 0, fc, ops);
+    if (g.sample) sample_ops(c, g, 0, fc, m);
+    else grating_ops(c, g, 0, fc, ops);
     cur = g.z_start + g.total_thickness();
   }
   auto final_hop = [&]() {
@@ -335,7 +416,9 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
         }
         cur = g.z_start;
         fc = fcs[i + 1];
-        grating_ops(c, g, g.x_positions.size() > 1 ? (int)step : 0, fc, ops);
+        const int st = g.x_positions.size() > 1 ? (int)step : 0;
+        if (g.sample) sample_ops(c, g, st, fc, m);
+        else grating_ops(c, g, st, fc, ops);
         cur = g.z_start + g.total_thickness();
       }
       final_hop();

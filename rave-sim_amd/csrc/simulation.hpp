@@ -11,13 +11,23 @@
 
 namespace rs {
 
+// An optical element: a grating (big-wave/optical_element.py:116-225) or,
+// with `sample` set, a sample grid (big-wave/optical_element.py:380-476:
+// grid[z][x] indexes db_list, 0 = vacuum; one propagation per grid row).
 struct GratingCfg {
   double pitch = 0, dc0 = 0.5, dc1 = 0.5, z_start = 0, thickness = 0;
   int nr_steps = 1;
   std::vector<double> x_positions{0.0};
   double substrate_thickness = 0;
   std::complex<double> db_a{0, 0}, db_b{0, 0}, db_s{0, 0};
-  double total_thickness() const { return thickness + substrate_thickness; }
+  bool sample = false;
+  double pixel_size_x = 0, pixel_size_z = 0;
+  int64_t rows = 0, cols = 0;
+  std::vector<int8_t> grid;                  // rows x cols
+  std::vector<std::complex<double>> db_list;  // [0] = vacuum
+  double total_thickness() const {
+    return sample ? pixel_size_z * (double)rows : thickness + substrate_thickness;
+  }
 };
 
 struct SourceCfg {
@@ -51,6 +61,11 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u);
 // Slice loop + substrate of one grating as member ops, using
 // x_positions[step] (big-wave/optical_element.py:134-206).
 void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops);
+
+// Row loop of a sample element as member ops (one SAMPLE mask and one
+// propagation over pixel_size_z per grid row), its pixel tables appended to
+// m.samples (big-wave/optical_element.py:432-476).
+void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m);
 
 // Check element ordering / phase-step consistency like big-wave's
 // check_simulation_inputs (big-wave/multisim.py:195-242); throws Error.

@@ -95,3 +95,26 @@ def cfg3_members(n_energies=64, N=2 ** 24, slices=32):
         db = complex(DB_SI_25KEV[0] * s ** 2, DB_SI_25KEV[1] * s ** 4)
         out.append((ek * 1000.0, float(wk), db))
     return out
+
+
+def sample_case(tmp: Path, N=2 ** 16, dtype="c16", steps=(0.0, 1.3e-7)):
+    """Point source at 30 keV, a Si phase grating, then a sample grid (int8,
+    6 rows x 40 pixels of 0.5 um: vacuum / Si / Au, 2 um per row) phase
+    stepped over `steps`, detector 50 mm behind.  Material boundaries inside
+    the grid give interpolated delta/beta (big-wave Sample.apply)."""
+    dx = 1e-9
+    rng = np.random.default_rng(11)
+    grid = rng.integers(0, 3, size=(6, 40)).astype(np.int8)
+    grid[:, :3] = 0
+    np.save(tmp / "sample_grid.npy", grid)
+    z_s = 0.012
+    els = [grating(1.0e-6, 0.01, 4e-6, 4, ["Si", 2.34]),
+           {"type": "sample", "z_start": z_s, "pixel_size_x": 0.5e-6, "pixel_size_z": 2e-6,
+            "grid_path": "sample_grid.npy", "materials": [["Si", 2.34], ["Au", 19.32]],
+            "x_positions": [float(s) for s in steps]}]
+    z_det = z_s + 6 * 2e-6 + 0.05
+    cfg = {"sim_params": sim_params(N, dx, z_det, 40e-6, 1e-6, 1.0),
+           "use_disk_vector": False, "save_final_u_vectors": False, "dtype": dtype, "elements": els}
+    subs = [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": 0.0},
+             "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
+    return cfg, subs, [math.pi / 2] * 3

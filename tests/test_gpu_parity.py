@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import bigwave as bw
-from simcases import DB_SI_25KEV, cfg1, cfg2
+from simcases import DB_SI_25KEV, cfg1, cfg2, sample_case
 
 pytestmark = pytest.mark.gpu
 
@@ -294,3 +294,22 @@ def test_every_fiber_length(ctx, dtype, lgR, lgA):
     bw.propagate(ref, dx, wl, dz, N, fc)
     tol = 1e-10 if dtype == np.complex128 else 2e-6
     assert rel_l2(got, ref.u) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [("c16", 1e-10), ("c8", 1e-4)])
+def test_sample_element_matches_oracle(ctx, tmp_path, dtype, tol):
+    """Sample grid rows (np.interp of delta + i beta between pixel centres,
+    zero outside the grid) with phase stepping, against the oracle."""
+    from ravesim.multisim import run_single_simulation
+    from ravesim.simdir import write_simdir
+    cfg, subs, ang = sample_case(tmp_path, dtype=dtype)
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    import shutil
+    shutil.copy(tmp_path / "sample_grid.npy", sd / "sample_grid.npy")
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    assert got.shape == ref.shape == (2, 40)
+    err = rel_l2(got, ref)
+    print(f"sample element {dtype} rel-L2 = {err:.3e}")
+    assert err < tol
+    assert rel_l2(got[0], got[1]) > 1e-3  # the two sample positions differ

@@ -117,7 +117,7 @@ deltabeta_table:
     (lambda c, s: c.__setitem__("dtype", "c32"), "Unknown dtype"),
     (lambda c, s: c["elements"][0].__setitem__("z_start", -1.0), "Overlapping"),
     (lambda c, s: s[0]["deltabeta_table"].clear(), "not found"),
-    (lambda c, s: c["elements"][0].__setitem__("type", "sample"), "not supported"),
+    (lambda c, s: c["elements"][0].__setitem__("type", "dba_grating"), "not supported"),
     (lambda c, s: c["sim_params"].__setitem__("N", 1000), "power of two"),
 ])
 def test_config_errors_are_reported(tmp_path, mutation, needle):
@@ -179,3 +179,38 @@ def test_setup_rules_match_oracle(tmp_path, golden_dir):
                                       [e.get_thickness() for e in els], cfg["sim_params"]["z_detector"],
                                       1e-6)
     assert np.allclose(comp["cutoff_angles"], ang, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("mutation,needle", [
+    (lambda c, g: g.__setitem__(0, np.full((2, 3), 3, dtype=np.int8)), "more materials"),
+    (lambda c, g: c["elements"][1].__setitem__("z_start", 0.0), "z_start"),
+    (lambda c, g: c["elements"][1].__setitem__("pixel_size_z", 0.0), "pixel sizes"),
+    (lambda c, g: g.__setitem__(0, np.zeros((2, 3), dtype=np.float64)), "int8"),
+])
+def test_sample_config_errors(tmp_path, mutation, needle):
+    """Sample elements are validated like big-wave's Sample.check_valid."""
+    from ravesim import _lib
+    from ravesim.multisim import validate_simdir
+    from ravesim.simdir import write_simdir
+    from simcases import sample_case
+    cfg, subs, ang = sample_case(tmp_path)
+    grid = [np.load(tmp_path / "sample_grid.npy")]
+    mutation(cfg, grid)
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    np.save(sd / "sample_grid.npy", grid[0])
+    with pytest.raises(_lib.RsError) as e:
+        validate_simdir(sd, 0)
+    assert needle in str(e.value)
+
+
+def test_sample_config_parses(tmp_path):
+    """A valid sample sim dir: one propagation per grid row, phase stepped."""
+    from ravesim.multisim import validate_simdir
+    from ravesim.simdir import write_simdir
+    from simcases import sample_case
+    cfg, subs, ang = sample_case(tmp_path)
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    np.save(sd / "sample_grid.npy", np.load(tmp_path / "sample_grid.npy"))
+    info = validate_simdir(sd, 0)
+    # N, c16, 2 phase steps, 40 px, 2 elements, 1 source prop + 4 slices + hop + 6 rows + hop
+    assert list(info) == [2 ** 16, 16, 2, 40, 2, 13]
