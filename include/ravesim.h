@@ -115,6 +115,25 @@ typedef struct rs_grating {
   double db_s_re, db_s_im;  /* substrate */
 } rs_grating;
 
+/* Double-pitch (DBA) grating: material A where both
+ * mod(x/pitch0 + dc0/2, 1) < dc0 and mod(x/pitch1, 1) < dc1 hold, each duty
+ * cycle interpolated linearly over the slices (big-wave/optical_element.py
+ * :227-338, 359-375; fast-wave/include/optical_element.hpp:50-63). */
+typedef struct rs_dba_grating {
+  double pitch0, pitch1;
+  double dc0[2];            /* first pitch's duty cycle at the first and after the last slice */
+  double dc1[2];            /* second pitch's duty cycle, likewise */
+  double z_start;
+  double thickness;         /* without substrate */
+  int32_t nr_steps;
+  int32_t nr_x_positions;
+  const double *x_positions;
+  double substrate_thickness;
+  double db_a_re, db_a_im;
+  double db_b_re, db_b_im;
+  double db_s_re, db_s_im;
+} rs_dba_grating;
+
 typedef struct rs_detector {
   double size;       /* detector_size [m] */
   double pixel_x;    /* detector_pixel_size_x */
@@ -149,6 +168,12 @@ int rs_propagate(rs_field *f, double wl, double dz, double cutoff_freq);
  * fast-wave/src/simulation.cpp:131-150), using x_positions[phase_step]. */
 int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step,
                      double cutoff_freq);
+
+/* DBA grating slices plus substrate (big-wave/optical_element.py:247-316,
+ * fast-wave/src/simulation.cpp:152-172 `apply_dbagrating`), using
+ * x_positions[phase_step]. */
+int rs_apply_dba_grating(rs_field *f, double wl, const rs_dba_grating *g, int phase_step,
+                         double cutoff_freq);
 
 /* u *= factor (fast-wave/fwcuda/kernels.cu:160-172). */
 int rs_scale(rs_field *f, double re, double im);

@@ -20,9 +20,14 @@ Pins (what checks each function independently of this file):
                          (25 pixels, atol 1e-14), plus closed-form checks
                          (free-space propagation of a Gaussian, unitarity,
                          composition of two hops) in tests/test_oracle_math.py
-  sample_apply           np.interp closed form on a one-row grid
-                         (tests/test_oracle_math.py); parity unpinned against
-                         big-wave's own output (no reference vector exercises it)
+  sample_apply           closed forms in tests/test_oracle_math.py: a one-pixel
+                         row is the hat function of linear interpolation, a
+                         uniform grid is the uniform grating slab, and moving
+                         x_position by s samples rolls the output by +s
+  dba_grating_mask/apply reductions to the plain grating (dc1 >= 1) and to a
+                         uniform slab (dc1 = 0), the second pitch's condition
+                         against exact rational arithmetic per slice, and the
+                         AND of both conditions (tests/test_oracle_dba.py)
 """
 from __future__ import annotations
 
@@ -66,6 +71,27 @@ class Grating:
 
     pitch: float
     dc: tuple
+    z_start: float
+    thickness: float
+    nr_steps: int
+    x_positions: np.ndarray
+    substrate_thickness: float
+    db_a: complex = 0j
+    db_b: complex = 0j
+    db_substrate: complex = 0j
+
+    def get_thickness(self) -> float:
+        return self.thickness + self.substrate_thickness
+
+
+@dataclass
+class DBAGrating:
+    """big-wave/optical_element.py:227-243 (deltabetas already resolved)."""
+
+    pitch0: float
+    pitch1: float
+    dc0: tuple
+    dc1: tuple
     z_start: float
     thickness: float
     nr_steps: int
@@ -285,11 +311,50 @@ def grating_apply(f: Fields, g: Grating, p: SimParams, cutoff_freq: float, step:
         propagate(f, p.dx, p.wl, g.substrate_thickness, p.chunk_size, cutoff_freq)
 
 
+def dba_grating_mask(dx: float, start: int, length: int, g: DBAGrating, z_fraction: float,
+                     step: int) -> np.ndarray:
+    """Material-A mask of one chunk of a DBA grating; big-wave/optical_element.py:359-375.
+
+    Both duty cycles are interpolated in FP64, x is the chunk's np.linspace as
+    for the plain grating, and the mask is
+    (mod(x/pitch0 + dc0/2, 1) < dc0) * (mod(x/pitch1, 1) < dc1).
+    """
+    dc0 = (1 - z_fraction) * g.dc0[0] + z_fraction * g.dc0[1]
+    dc1 = (1 - z_fraction) * g.dc1[0] + z_fraction * g.dc1[1]
+    x_start = start * dx + g.x_positions[step]
+    x_stop = x_start + length * dx
+    x = np.linspace(x_start, x_stop, length, endpoint=False)
+    phase0 = np.mod(x / g.pitch0 + dc0 / 2, 1)
+    phase1 = np.mod(x / g.pitch1, 1)
+    return (phase0 < dc0) * (phase1 < dc1)
+
+
+def dba_grating_apply(f: Fields, g: DBAGrating, p: SimParams, cutoff_freq: float,
+                      step: int) -> None:
+    """Slice loop of a DBA grating plus its substrate; big-wave/optical_element.py:247-316.
+    Factor A where the mask holds, B elsewhere (np.place with mask and 1 - mask)."""
+    z_step = g.thickness / g.nr_steps
+    fa = material_factor(g.db_a, z_step, p.wl)
+    fb = material_factor(g.db_b, z_step, p.wl)
+    for i in range(g.nr_steps):
+        zf = i / g.nr_steps
+        for idx in range(0, p.N, p.chunk_size):
+            length = min(p.chunk_size, p.N - idx)
+            mask = dba_grating_mask(p.dx, idx - p.N // 2, length, g, zf, step)
+            fac = np.where(mask, fa, fb).astype(f.u.dtype)
+            f.u[idx:idx + length] *= fac
+        propagate(f, p.dx, p.wl, z_step, p.chunk_size, cutoff_freq)
+    if g.substrate_thickness > 0:
+        f.u *= np.complex128(material_factor(g.db_substrate, g.substrate_thickness, p.wl))
+        propagate(f, p.dx, p.wl, g.substrate_thickness, p.chunk_size, cutoff_freq)
+
+
 def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: int) -> None:
     """Row-by-row sample; big-wave/optical_element.py:432-476.
 
-    Each z-row's delta+i*beta is linearly interpolated (np.interp, zero outside)
-    from pixel positions arange(len)*px + x_pos - len*px/2 onto the field x
+    Each z-row's delta+i*beta is linearly interpolated (np.interp over the
+    complex values, zero outside; numpy's complex interp forms the slope as
+    difference * (1 / dx)) from pixel positions arange(len)*px + x_pos - len*px/2 onto the field x
     coordinates, then exp(2 pi i dz/wl db) is applied and the row propagated.
     """
     rowlen = s.grid.shape[1]
@@ -300,8 +365,7 @@ def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: i
             length = min(p.chunk_size, p.N - idx)
             start = idx - p.N / 2
             xc = np.arange(start, start + length) * p.dx
-            db = (np.interp(xc, row_x, row_db.real, left=0.0, right=0.0)
-                  + 1j * np.interp(xc, row_x, row_db.imag, left=0.0, right=0.0))
+            db = np.interp(xc, row_x, row_db, left=0.0, right=0.0)
             f.u[idx:idx + length] *= material_factor(db, s.pixel_size_z, p.wl)
         propagate(f, p.dx, p.wl, s.pixel_size_z, p.chunk_size, cutoff_freq)
 
@@ -309,6 +373,8 @@ def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: i
 def element_apply(f: Fields, el, p: SimParams, cutoff_freq: float, step: int) -> None:
     if isinstance(el, Grating):
         grating_apply(f, el, p, cutoff_freq, step)
+    elif isinstance(el, DBAGrating):
+        dba_grating_apply(f, el, p, cutoff_freq, step)
     elif isinstance(el, Sample):
         sample_apply(f, el, p, cutoff_freq, step)
     else:
@@ -474,6 +540,18 @@ def elements_from_config(els: list, table: Sequence, base_dir: Optional[Path] = 
                                substrate_thickness=float(d["substrate_thickness"]),
                                db_a=db(d.get("mat_a")), db_b=db(d.get("mat_b")),
                                db_substrate=db(d.get("mat_substrate"))))
+        elif d["type"] == "dba_grating":
+            # keys of fast-wave/src/config_parsing.cpp:107-137 (big-wave's
+            # config.py:103-116 names the same keys)
+            out.append(DBAGrating(pitch0=float(d["pitch0"]), pitch1=float(d["pitch1"]),
+                                  dc0=(float(d["dc0"][0]), float(d["dc0"][1])),
+                                  dc1=(float(d["dc1"][0]), float(d["dc1"][1])),
+                                  z_start=float(d["z_start"]), thickness=float(d["thickness"]),
+                                  nr_steps=int(d["nr_steps"]),
+                                  x_positions=np.array(d["x_positions"], dtype=np.float64),
+                                  substrate_thickness=float(d["substrate_thickness"]),
+                                  db_a=db(d.get("mat_a")), db_b=db(d.get("mat_b")),
+                                  db_substrate=db(d.get("mat_substrate"))))
         elif d["type"] == "sample":
             grid = np.load(Path(base_dir) / d["grid_path"]) if base_dir is not None else d["grid"]
             assert grid.dtype == np.int8

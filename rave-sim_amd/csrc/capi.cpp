@@ -101,6 +101,26 @@ rs::GratingCfg grating_from(const rs_grating *g) {
   return c;
 }
 
+rs::GratingCfg dba_grating_from(const rs_dba_grating *g) {
+  rs::GratingCfg c;
+  c.dba = true;
+  c.pitch = g->pitch0;
+  c.pitch1 = g->pitch1;
+  c.dc0 = g->dc0[0];
+  c.dc1 = g->dc0[1];
+  c.dcb0 = g->dc1[0];
+  c.dcb1 = g->dc1[1];
+  c.z_start = g->z_start;
+  c.thickness = g->thickness;
+  c.nr_steps = g->nr_steps;
+  c.x_positions.assign(g->x_positions, g->x_positions + g->nr_x_positions);
+  c.substrate_thickness = g->substrate_thickness;
+  c.db_a = {g->db_a_re, g->db_a_im};
+  c.db_b = {g->db_b_re, g->db_b_im};
+  c.db_s = {g->db_s_re, g->db_s_im};
+  return c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -325,6 +345,22 @@ int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step
   return guarded([&] {
     rs::SimConfig c = field_cfg(f, wl);
     const rs::GratingCfg gc = grating_from(g);
+    rs::Member m;
+    m.wl = wl;
+    rs::grating_ops(c, gc, phase_step, cutoff_freq, m.ops);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
+int rs_apply_dba_grating(rs_field *f, double wl, const rs_dba_grating *g, int phase_step, double cutoff_freq) {
+  if (!f || !g) return fail(RS_ERR_INVALID, "NULL argument");
+  if (g->nr_steps < 1 || g->nr_x_positions < 1 || !g->x_positions)
+    return fail(RS_ERR_INVALID, "grating needs nr_steps >= 1 and at least one x position");
+  if (phase_step < 0 || phase_step >= g->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  return guarded([&] {
+    rs::SimConfig c = field_cfg(f, wl);
+    const rs::GratingCfg gc = dba_grating_from(g);
     rs::Member m;
     m.wl = wl;
     rs::grating_ops(c, gc, phase_step, cutoff_freq, m.ops);

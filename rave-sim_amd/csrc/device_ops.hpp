@@ -296,6 +296,39 @@ __device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *
 // big-wave bit for bit: x = linspace(x0, x0 + len*dx, len, endpoint=False)
 // per chunk, mask = mod(x/pitch + dc/2, 1) < dc, all in FP64 without
 // contraction (big-wave/optical_element.py:341-357).
+// Grating coordinate of sample n as big-wave forms it: np.linspace per chunk
+// (big-wave/optical_element.py:352-355), without contraction.
+__device__ __forceinline__ double grating_x(const XOp &op, int64_t n, int64_t N) {
+#pragma clang fp contract(off)
+  if (op.single_chunk) return (double)n * op.xstep + op.x0;
+  const int64_t c = n / op.chunk;
+  const int64_t k = n - c * op.chunk;
+  int64_t len = N - c * op.chunk;
+  if (len > op.chunk) len = op.chunk;
+  const int64_t start = c * op.chunk - N / 2;
+  const double x_start = (double)start * op.dx + op.xpos;
+  const double x_stop = x_start + (double)len * op.dx;
+  const double step = (x_stop - x_start) / (double)len;
+  return (double)k * step + x_start;
+}
+
+// np.mod(x / pitch + off, 1) as big-wave rounds it.  x * (1/pitch) differs
+// from the correctly rounded quotient by a few ulp of max(|t|, 1); outside a
+// guard band around the mask edges (ph = 0, ph = dc and ph = 1) that cannot
+// change the decision ph < dc, inside it the exact quotient is used.
+__device__ __forceinline__ double grating_phase(double x, double pitch, double inv_pitch, double off,
+                                                double dc) {
+#pragma clang fp contract(off)
+  double t = x * inv_pitch + off;
+  double ph = t - floor(t);
+  const double guard = (fabs(t) + 1.0) * 1e-13;
+  if (ph < guard || ph > 1.0 - guard || fabs(ph - dc) < guard) {
+    t = x / pitch + off;
+    ph = t - floor(t);
+  }
+  return ph;
+}
+
 template <typename T2>
 __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 v) {
 #pragma clang fp contract(off)
@@ -304,33 +337,24 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
   bool mul = true;
   switch (op.kind) {
     case XOP_MASK: {
-      double x;
-      if (op.single_chunk) {
-        x = (double)n * op.xstep + op.x0;
+      const double x = grating_x(op, n, N);
+      if (grating_phase(x, op.pitch, op.inv_pitch, op.half_dc, op.dc) < op.dc) {
+        re = op.fa[0]; im = op.fa[1];
       } else {
-        const int64_t c = n / op.chunk;
-        const int64_t k = n - c * op.chunk;
-        int64_t len = N - c * op.chunk;
-        if (len > op.chunk) len = op.chunk;
-        const int64_t start = c * op.chunk - N / 2;
-        const double x_start = (double)start * op.dx + op.xpos;
-        const double x_stop = x_start + (double)len * op.dx;
-        const double step = (x_stop - x_start) / (double)len;
-        x = (double)k * step + x_start;
+        re = op.fb[0]; im = op.fb[1];
       }
-      // t = x/pitch + dc/2 as big-wave rounds it.  x * (1/pitch) differs from
-      // the correctly rounded quotient by a few ulp of max(|t|, 1); outside a
-      // guard band around the mask edges (ph = 0 and ph = dc) that cannot
-      // change the decision, inside it the exact quotient is used.
-      double t = x * op.inv_pitch + op.half_dc;
-      double ph = t - floor(t);
-      const double guard = (fabs(t) + 1.0) * 1e-13;
-      if (ph < guard || ph > 1.0 - guard || fabs(ph - op.dc) < guard) {
-        t = x / op.pitch + op.half_dc;
-        ph = t - floor(t);
+      break;
+    }
+    case XOP_DBA: {
+      // (mod(x/pitch0 + dc0/2, 1) < dc0) * (mod(x/pitch1, 1) < dc1)
+      // (big-wave/optical_element.py:359-375)
+      const double x = grating_x(op, n, N);
+      if (grating_phase(x, op.pitch, op.inv_pitch, op.half_dc, op.dc) < op.dc &&
+          grating_phase(x, op.pitch1, op.inv_pitch1, 0.0, op.dc1) < op.dc1) {
+        re = op.fa[0]; im = op.fa[1];
+      } else {
+        re = op.fb[0]; im = op.fb[1];
       }
-      if (ph < op.dc) { re = op.fa[0]; im = op.fa[1]; }
-      else { re = op.fb[0]; im = op.fb[1]; }
       break;
     }
     case XOP_SCALAR:
@@ -353,8 +377,8 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
       mul = false;
       break;
     case XOP_SAMPLE: {
-      // delta + i beta by np.interp(x, rx, db, left=0, right=0) per
-      // component, then exp(2 pi i dz / wl * db) (big-wave/optical_element.py
+      // delta + i beta by np.interp(x, rx, db, left=0, right=0) over the
+      // complex values, then exp(2 pi i dz / wl * db) (big-wave/optical_element.py
       // :449-470); intervals between equal pixels and exact knots use the
       // host-computed pixel factor
       const double x = ((double)n - (double)N / 2.0) * op.dx;
@@ -379,9 +403,10 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
         im = op.s_f[2 * j + 1];
         break;
       }
-      const double span = rx[j + 1] - rx[j];
-      const double dr = ((db[2 * j + 2] - db[2 * j]) / span) * (x - rx[j]) + db[2 * j];
-      const double di = ((db[2 * j + 3] - db[2 * j + 1]) / span) * (x - rx[j]) + db[2 * j + 1];
+      // numpy's complex interp: slope = difference * (1 / dx), no fused ops
+      const double inv = 1.0 / (rx[j + 1] - rx[j]);
+      const double dr = ((db[2 * j + 2] - db[2 * j]) * inv) * (x - rx[j]) + db[2 * j];
+      const double di = ((db[2 * j + 3] - db[2 * j + 1]) * inv) * (x - rx[j]) + db[2 * j + 1];
       const double er = -(op.s_a * di) + 0.0 * dr;
       const double ei = op.s_a * dr + 0.0 * di;
       const double e = exp(er);

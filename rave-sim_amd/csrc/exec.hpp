@@ -89,7 +89,7 @@ inline XOp scalar_xop(double re, double im) {
   return x;
 }
 
-// x-space op composition: first `a`, then `b` (b is a MASK, SAMPLE or SCALAR)
+// x-space op composition: first `a`, then `b` (b is a MASK, DBA, SAMPLE or SCALAR)
 inline XOp compose(const XOp &a, const XOp &b) {
   if (a.kind == XOP_NONE) return b;
   XOp r = a;
@@ -103,7 +103,8 @@ inline XOp compose(const XOp &a, const XOp &b) {
     r.post[1] = p1;
     return r;
   }
-  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) && (b.kind == XOP_MASK || b.kind == XOP_SAMPLE)) {
+  if ((a.kind == XOP_SCALAR || a.kind == XOP_ONES) &&
+      (b.kind == XOP_MASK || b.kind == XOP_DBA || b.kind == XOP_SAMPLE)) {
     XOp m = b;
     // value of `a` at every sample: fa * post for a scalar, post for ones
     const double s0 = a.kind == XOP_ONES ? a.post[0] : a.fa[0] * a.post[0] - a.fa[1] * a.post[1];
@@ -126,7 +127,7 @@ inline XOp compose(const XOp &a, const XOp &b) {
 // (|x0| + 2 N |xstep|) / pitch, so m_guard (1e-13 of that plus 1) bounds it
 // with a wide margin.
 inline void finalize_xop(XOp &x, int64_t N) {
-  if (x.kind != XOP_MASK && x.kind != XOP_SCALAR) return;
+  if (x.kind != XOP_MASK && x.kind != XOP_DBA && x.kind != XOP_SCALAR) return;
   auto fold = [&](double *f) {
     const double r = f[0] * x.post[0] - f[1] * x.post[1];
     const double i = f[0] * x.post[1] + f[1] * x.post[0];
@@ -134,7 +135,7 @@ inline void finalize_xop(XOp &x, int64_t N) {
     f[1] = i;
   };
   fold(x.fa);
-  if (x.kind == XOP_MASK) fold(x.fb);
+  if (x.kind != XOP_SCALAR) fold(x.fb);
   x.post[0] = 1.0;
   x.post[1] = 0.0;
   if (x.kind == XOP_MASK && x.single_chunk) {

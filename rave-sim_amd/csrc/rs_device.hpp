@@ -22,6 +22,7 @@ enum XopKind : int32_t {
   XOP_POINT = 3,   // u = exp(i kphase r) / sqrt(r)   (generated, input not read)
   XOP_ONES = 4,    // u = 1 (plane wave, generated)
   XOP_SAMPLE = 5,  // sample row: u *= exp(i a interp(x; rx, db)) (big-wave Sample.apply)
+  XOP_DBA = 6,     // DBA grating: frac(x/pitch + dc/2) < dc and frac(x/pitch1) < dc1 ? fa : fb
 };
 
 // x-space (natural order) elementwise operation.  Every kind is followed by a
@@ -38,6 +39,9 @@ struct XOp {
   double inv_pitch;       // 1/pitch for the fast mask test (exact division near edges)
   double x0, xstep;       // single chunk: linspace start and step, computed on the host
   int64_t chunk;
+  // second pitch and duty cycle of a DBA grating
+  // (big-wave/optical_element.py:359-375)
+  double pitch1, inv_pitch1, dc1;
   // point source (big-wave/propagation.py:141-151)
   double xs, z, kphase;
   // fast mask test (single chunk), set by finalize_xop on the host:

@@ -127,12 +127,24 @@ SimConfig parse_simdir(const std::string &sim_dir, int source_idx) {
       c.elements.push_back(g);
       continue;
     }
-    if (type != "grating")
-      throw Error(6, "optical element type '" + type + "' is not supported by this build (grating, sample)");
+    if (type != "grating" && type != "dba_grating")
+      throw Error(6, "optical element type '" + type +
+                         "' is not supported by this build (grating, dba_grating, sample)");
     GratingCfg g;
-    g.pitch = get_f(e, "pitch");
-    g.dc0 = e["dc"][0].as_double("dc.0");
-    g.dc1 = e["dc"][1].as_double("dc.1");
+    if (type == "dba_grating") {
+      // fast-wave/src/config_parsing.cpp:107-137
+      g.dba = true;
+      g.pitch = get_f(e, "pitch0");
+      g.pitch1 = get_f(e, "pitch1");
+      g.dc0 = e["dc0"][0].as_double("dc0.0");
+      g.dc1 = e["dc0"][1].as_double("dc0.1");
+      g.dcb0 = e["dc1"][0].as_double("dc1.0");
+      g.dcb1 = e["dc1"][1].as_double("dc1.1");
+    } else {
+      g.pitch = get_f(e, "pitch");
+      g.dc0 = e["dc"][0].as_double("dc.0");
+      g.dc1 = e["dc"][1].as_double("dc.1");
+    }
     g.z_start = get_f(e, "z_start");
     g.thickness = get_f(e, "thickness");
     g.nr_steps = (int)e["nr_steps"].as_int("nr_steps");
@@ -179,6 +191,8 @@ void check_config(const SimConfig &c) {
       throw Error(2, "grating thickness must be positive");
     }
     if (g.substrate_thickness < 0) throw Error(2, "substrate thickness must be >= 0");
+    // big-wave DBAGrating.check_valid (optical_element.py:318-320)
+    if (g.dba && !(g.thickness > 0)) throw Error(2, "DBA grating thickness must be positive");
     if (!g.sample && g.nr_steps < 1) throw Error(2, "nr_steps must be >= 1");
     if (g.x_positions.empty()) throw Error(2, "x_positions must not be empty");
     if (g.x_positions.size() > 1) {
@@ -214,6 +228,13 @@ XOp mask_op(const SimConfig &c, const GratingCfg &g, int slice, int step, const 
   x.chunk = c.chunk_size;
   x.single_chunk = c.N <= c.chunk_size;
   x.inv_pitch = 1.0 / g.pitch;
+  if (g.dba) {
+    // big-wave/optical_element.py:368-369: both duty cycles interpolated in FP64
+    x.kind = XOP_DBA;
+    x.pitch1 = g.pitch1;
+    x.inv_pitch1 = 1.0 / g.pitch1;
+    x.dc1 = (1.0 - zf) * g.dcb0 + zf * g.dcb1;
+  }
   // np.linspace(x_start, x_start + N dx, N, endpoint=False) of the single chunk
   // (big-wave/optical_element.py:352-355), evaluated as numpy does
   const double x_start = (double)(-(c.N / 2)) * c.dx + x.xpos;

@@ -20,6 +20,11 @@ struct GratingCfg {
   std::vector<double> x_positions{0.0};
   double substrate_thickness = 0;
   std::complex<double> db_a{0, 0}, db_b{0, 0}, db_s{0, 0};
+  // DBA grating (fast-wave/include/optical_element.hpp:50-63): material A
+  // where both the (pitch, dc0..dc1) grating and frac(x/pitch1) < dc1 hold;
+  // pitch / dc0 / dc1 above are its pitch0 and dc0 pair, dcb its dc1 pair
+  bool dba = false;
+  double pitch1 = 0, dcb0 = 0, dcb1 = 0;
   bool sample = false;
   double pixel_size_x = 0, pixel_size_z = 0;
   int64_t rows = 0, cols = 0;

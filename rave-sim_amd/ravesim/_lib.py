@@ -31,7 +31,7 @@ EXPORTS = (
     "rs_version", "rs_last_error", "rs_ctx_create", "rs_ctx_destroy", "rs_ctx_synchronize",
     "rs_run_simdir", "rs_validate_simdir", "rs_field_create", "rs_field_destroy",
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
-    "rs_apply_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
+    "rs_apply_grating", "rs_apply_dba_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
     "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
     "rs_server_destroy",
@@ -59,6 +59,16 @@ class rs_grating(C.Structure):
                 ("z_start", C.c_double), ("thickness", C.c_double), ("nr_steps", C.c_int32),
                 ("nr_x_positions", C.c_int32), ("x_positions", C.POINTER(C.c_double)),
                 ("substrate_thickness", C.c_double),
+                ("db_a_re", C.c_double), ("db_a_im", C.c_double),
+                ("db_b_re", C.c_double), ("db_b_im", C.c_double),
+                ("db_s_re", C.c_double), ("db_s_im", C.c_double)]
+
+
+class rs_dba_grating(C.Structure):
+    _fields_ = [("pitch0", C.c_double), ("pitch1", C.c_double), ("dc0", C.c_double * 2),
+                ("dc1", C.c_double * 2), ("z_start", C.c_double), ("thickness", C.c_double),
+                ("nr_steps", C.c_int32), ("nr_x_positions", C.c_int32),
+                ("x_positions", C.POINTER(C.c_double)), ("substrate_thickness", C.c_double),
                 ("db_a_re", C.c_double), ("db_a_im", C.c_double),
                 ("db_b_re", C.c_double), ("db_b_im", C.c_double),
                 ("db_s_re", C.c_double), ("db_s_im", C.c_double)]
@@ -116,6 +126,8 @@ def _load() -> C.CDLL:
         "rs_point_source": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
         "rs_propagate": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
         "rs_apply_grating": (C.c_int, [P, C.c_double, C.POINTER(rs_grating), C.c_int, C.c_double]),
+        "rs_apply_dba_grating": (C.c_int, [P, C.c_double, C.POINTER(rs_dba_grating), C.c_int,
+                                           C.c_double]),
         "rs_scale": (C.c_int, [P, C.c_double, C.c_double]),
         "rs_square_and_downsample": (C.c_int, [P, C.POINTER(rs_detector), C.c_double,
                                                C.POINTER(C.c_double), C.c_int64]),

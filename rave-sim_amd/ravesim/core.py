@@ -113,6 +113,43 @@ class Grating:
         return self.thickness + self.substrate_thickness
 
 
+@dataclass
+class DBAGrating:
+    """Double-pitch grating, fields as in big-wave/optical_element.py:227-243
+    with deltabeta values resolved (0 = vacuum)."""
+
+    pitch0: float
+    pitch1: float
+    dc0: tuple
+    dc1: tuple
+    z_start: float
+    thickness: float
+    nr_steps: int
+    x_positions: Sequence[float] = (0.0,)
+    substrate_thickness: float = 0.0
+    db_a: complex = 0j
+    db_b: complex = 0j
+    db_substrate: complex = 0j
+
+    def to_c(self):
+        xs = (C.c_double * len(self.x_positions))(*[float(v) for v in self.x_positions])
+        g = _lib.rs_dba_grating(
+            pitch0=self.pitch0, pitch1=self.pitch1,
+            dc0=(C.c_double * 2)(*[float(v) for v in self.dc0]),
+            dc1=(C.c_double * 2)(*[float(v) for v in self.dc1]),
+            z_start=self.z_start, thickness=self.thickness,
+            nr_steps=self.nr_steps, nr_x_positions=len(self.x_positions),
+            x_positions=C.cast(xs, C.POINTER(C.c_double)),
+            substrate_thickness=self.substrate_thickness,
+            db_a_re=complex(self.db_a).real, db_a_im=complex(self.db_a).imag,
+            db_b_re=complex(self.db_b).real, db_b_im=complex(self.db_b).imag,
+            db_s_re=complex(self.db_substrate).real, db_s_im=complex(self.db_substrate).imag)
+        return g, xs
+
+    def get_thickness(self) -> float:
+        return self.thickness + self.substrate_thickness
+
+
 class Field:
     """A device-resident complex field (rs_field) with big-wave's operators."""
 
@@ -166,6 +203,12 @@ class Field:
                       cutoff_freq: float = math.inf) -> None:
         cg, keep = g.to_c()
         check(lib.rs_apply_grating(self._h, wl, C.byref(cg), phase_step, cutoff_freq))
+        del keep
+
+    def apply_dba_grating(self, wl: float, g: DBAGrating, phase_step: int = 0,
+                          cutoff_freq: float = math.inf) -> None:
+        cg, keep = g.to_c()
+        check(lib.rs_apply_dba_grating(self._h, wl, C.byref(cg), phase_step, cutoff_freq))
         del keep
 
     def scale(self, factor: complex) -> None:

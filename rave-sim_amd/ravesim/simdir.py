@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import copy
 import math
+import shutil
 from pathlib import Path
 from typing import Callable, Optional, Sequence, Tuple, Union
 
@@ -57,13 +58,32 @@ def cutoff_angles(detector_size: float, dx: float, energy_range, z_source: float
 
 
 def _materials(elements) -> list:
+    """Deduplicated materials in element order (big-wave/optical_element.py:513-545)."""
     mats = []
+
+    def push(m):
+        if m is not None and [m[0], float(m[1])] not in mats:
+            mats.append([m[0], float(m[1])])
+
     for el in elements:
-        for k in ("mat_a", "mat_b", "mat_substrate"):
-            m = el.get(k)
-            if m is not None and [m[0], float(m[1])] not in mats:
-                mats.append([m[0], float(m[1])])
+        if el["type"] == "sample":
+            for m in el["materials"]:
+                push(m)
+        else:
+            for k in ("mat_a", "mat_b", "mat_substrate"):
+                push(el.get(k))
     return mats
+
+
+def _thickness(el: dict, config_dir: Optional[Path]) -> float:
+    """get_thickness() of a grating, DBA grating or sample element
+    (big-wave/optical_element.py:132, 245, 420)."""
+    if el["type"] == "sample":
+        if config_dir is None:
+            raise ValueError("a sample element needs config_dir to find its grid")
+        grid = np.load(Path(config_dir) / el["grid_path"])
+        return float(el["pixel_size_z"]) * grid.shape[0]
+    return float(el["thickness"]) + float(el.get("substrate_thickness", 0.0))
 
 
 def _table(materials, energy: float, deltabeta: DeltaBeta) -> list:
@@ -95,9 +115,11 @@ def write_simdir(sim_dir: Union[str, Path], config: dict, sources: Sequence[dict
     return sim_dir
 
 
-def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBeta) -> Path:
+def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBeta,
+                     config_dir: Optional[Union[str, Path]] = None) -> Path:
     """Point-source multisource setup (big-wave/multisim.py:282-521, "points"
-    branch) into `sim_dir`."""
+    branch) into `sim_dir`.  Sample grids are read relative to `config_dir`
+    and copied into the sim dir under their base names (multisim.py:489-499)."""
     cfg = copy.deepcopy(config)
     sp = cfg["sim_params"]
     N, dx = int(sp["N"]), float(sp["dx"])
@@ -122,7 +144,7 @@ def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBe
     subs = [{"source": {"type": "point", "x": float(xs[i]), "z": z_src},
              "energy": float(energies[i]),
              "deltabeta_table": _table(mats, float(energies[i]), deltabeta)} for i in range(n)]
-    heights = [float(e["thickness"]) + float(e.get("substrate_thickness", 0.0)) for e in els]
+    heights = [_thickness(e, config_dir) for e in els]
     angles, max_x_list = cutoff_angles(float(sp["detector_size"]), dx, e_rng, z_src,
                                        [float(e["z_start"]) for e in els], heights,
                                        float(sp["z_detector"]), max(abs(x_rng[0]), abs(x_rng[1])))
@@ -130,5 +152,13 @@ def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBe
     if N < required_N:
         raise ValueError(f"Reflections at the boundary might occur: N={N}, required_N={required_N}")
     points = [{"x": float(xs[i]), "z": z_src, "energy": float(energies[i])} for i in range(n)]
+    grids = []
+    for el in els:
+        if el["type"] == "sample":
+            grids.append(Path(config_dir) / el["grid_path"])
+            el["grid_path"] = Path(el["grid_path"]).name
+    Path(sim_dir).mkdir(parents=True, exist_ok=True)
+    for gf in grids:
+        shutil.copyfile(gf, Path(sim_dir) / gf.name)
     return write_simdir(sim_dir, cfg, subs, angles,
                         {"max_x": max_x_list[-1], "energy_range": list(e_rng), "source_points": points})

@@ -118,3 +118,22 @@ def sample_case(tmp: Path, N=2 ** 16, dtype="c16", steps=(0.0, 1.3e-7)):
     subs = [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": 0.0},
              "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
     return cfg, subs, [math.pi / 2] * 3
+
+
+def dba_case(N=2 ** 16, dtype="c16", steps=(0.0, 1.05e-6), chunk=None):
+    """Point source at 30 keV, a DBA grating (Au, pitches 2.1 and 6.3 um, both
+    duty cycles graded over 6 slices, Si substrate) phase stepped over `steps`,
+    then a plain Si grating and the detector.  pitch/dx = 2100 and 6300 put
+    samples exactly on both conditions' edges."""
+    dx = 1e-9
+    dba = {"type": "dba_grating", "pitch0": 2.1e-6, "pitch1": 6.3e-6, "dc0": [0.45, 0.55],
+           "dc1": [0.6, 0.7], "z_start": 0.01, "thickness": 6e-6, "nr_steps": 6,
+           "x_positions": [float(s) for s in steps], "substrate_thickness": 2e-6,
+           "mat_a": ["Au", 19.32], "mat_b": None, "mat_substrate": ["Si", 2.34]}
+    els = [dba, grating(1.0e-6, 0.02, 4e-6, 4, ["Si", 2.34])]
+    z_det = 0.05
+    cfg = {"sim_params": sim_params(N, dx, z_det, 40e-6, 1e-6, 1.0, chunk),
+           "use_disk_vector": False, "save_final_u_vectors": False, "dtype": dtype, "elements": els}
+    subs = [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": 0.0},
+             "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
+    return cfg, subs, [math.pi / 2] * 3

@@ -313,3 +313,54 @@ def test_sample_element_matches_oracle(ctx, tmp_path, dtype, tol):
     print(f"sample element {dtype} rel-L2 = {err:.3e}")
     assert err < tol
     assert rel_l2(got[0], got[1]) > 1e-3  # the two sample positions differ
+
+
+@pytest.mark.parametrize("N,chunk,xpos,dtype,plain", [
+    (2 ** 14, None, 0.0, np.complex64, False), (2 ** 16, 1000, 3.3e-7, np.complex128, False),
+    (2 ** 20, None, -1.05e-6, np.complex64, False), (2 ** 16, None, 2.1e-7, np.complex128, True)])
+def test_dba_grating_mask_bit_exact(ctx, N, chunk, xpos, dtype, plain):
+    """DBA grating decisions, one slice at a time: pitch0/dx = 2100 and
+    pitch1/dx = 6300 put samples exactly on both conditions' edges."""
+    import ravesim
+    dx, wl = 2e-9, 2.7e-11
+    t = 1e-12  # negligible propagation; material A halves the amplitude
+    beta = math.log(2) * wl / (2 * math.pi * t)
+    chunk = chunk or N
+    og = bw.DBAGrating(pitch0=4.2e-6, pitch1=12.6e-6, dc0=(0.37, 0.62), dc1=(0.55, 0.3),
+                       z_start=0.0, thickness=t * 3, nr_steps=3,
+                       x_positions=np.array([xpos, xpos + 1e-7]), substrate_thickness=0.0)
+    for i in range(3):
+        zf = i / 3
+        dc0 = (1 - zf) * og.dc0[0] + zf * og.dc0[1]
+        dc1 = (1 - zf) * og.dc1[0] + zf * og.dc1[1]
+        g1 = ravesim.DBAGrating(pitch0=4.2e-6, pitch1=12.6e-6, dc0=(dc0, dc0), dc1=(dc1, dc1),
+                                z_start=0.0, thickness=t, nr_steps=1,
+                                x_positions=[xpos, xpos + 1e-7], db_a=complex(0, beta))
+        f = ravesim.Field(ctx, N, dx, dtype=dtype, chunk_size=chunk, plain=plain)
+        f.upload(np.ones(N))
+        f.apply_dba_grating(wl, g1, phase_step=1)
+        got = np.abs(f.download()) < 0.75
+        f.close()
+        exp = np.concatenate([bw.dba_grating_mask(dx, k - N // 2, min(chunk, N - k), og, zf, 1)
+                              for k in range(0, N, chunk)])
+        assert got.shape == exp.shape
+        assert np.array_equal(got, exp), f"slice {i}"
+
+
+@pytest.mark.parametrize("dtype,tol,chunk", [("c16", 1e-10, None), ("c16", 1e-10, 3000),
+                                             ("c8", 1e-4, None)])
+def test_dba_grating_simdir_matches_oracle(ctx, tmp_path, dtype, tol, chunk):
+    """A phase-stepped DBA grating with substrate through the sim-dir entry
+    point (fast-wave's `dba_grating` element) against the oracle."""
+    from ravesim.multisim import run_single_simulation
+    from ravesim.simdir import write_simdir
+    from simcases import dba_case
+    cfg, subs, ang = dba_case(dtype=dtype, chunk=chunk)
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    assert got.shape == ref.shape == (2, 40)
+    err = rel_l2(got, ref)
+    print(f"DBA grating {dtype} chunk {chunk} rel-L2 = {err:.3e}")
+    assert err < tol
+    assert rel_l2(got[0], got[1]) > 1e-3  # the two grating positions differ

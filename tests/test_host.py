@@ -8,6 +8,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+import yaml
 
 from oracle import bigwave as bw
 from simcases import cfg1, cfg2
@@ -117,7 +118,7 @@ deltabeta_table:
     (lambda c, s: c.__setitem__("dtype", "c32"), "Unknown dtype"),
     (lambda c, s: c["elements"][0].__setitem__("z_start", -1.0), "Overlapping"),
     (lambda c, s: s[0]["deltabeta_table"].clear(), "not found"),
-    (lambda c, s: c["elements"][0].__setitem__("type", "dba_grating"), "not supported"),
+    (lambda c, s: c["elements"][0].__setitem__("type", "lens"), "not supported"),
     (lambda c, s: c["sim_params"].__setitem__("N", 1000), "power of two"),
 ])
 def test_config_errors_are_reported(tmp_path, mutation, needle):
@@ -214,3 +215,41 @@ def test_sample_config_parses(tmp_path):
     info = validate_simdir(sd, 0)
     # N, c16, 2 phase steps, 40 px, 2 elements, 1 source prop + 4 slices + hop + 6 rows + hop
     assert list(info) == [2 ** 16, 16, 2, 40, 2, 13]
+
+
+def test_dba_config_parses(tmp_path):
+    """fast-wave's dba_grating element (config_parsing.cpp:107-137): 6 slices
+    + substrate, phase stepped over two positions."""
+    from ravesim.multisim import validate_simdir
+    from ravesim.simdir import write_simdir
+    from simcases import dba_case
+    cfg, subs, ang = dba_case()
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    assert list(validate_simdir(sd, 0))[:4] == [2 ** 16, 16, 2, 40]
+    cfg["elements"][0]["thickness"] = 0.0
+    sd = write_simdir(tmp_path / "bad", cfg, subs, ang)
+    from ravesim import _lib
+    with pytest.raises(_lib.RsError) as e:
+        validate_simdir(sd, 0)
+    assert "thickness" in str(e.value)
+
+
+def test_setup_simulation_with_sample_and_dba(tmp_path):
+    """The sim-dir producer collects sample materials, takes a sample's
+    thickness from its grid and copies the grid (big-wave/multisim.py:489-499)."""
+    from ravesim.multisim import validate_simdir
+    from ravesim.simdir import setup_simulation
+    from simcases import dba_case, sample_case
+    cfg, _, _ = sample_case(tmp_path)
+    dcfg, _, _ = dba_case()
+    cfg["elements"].insert(1, dict(dcfg["elements"][0], z_start=0.011))
+    cfg["multisource"] = {"type": "points", "nr_source_points": 2, "energy_range": [29000.0, 31000.0],
+                          "x_range": [-1e-7, 1e-7], "z": -0.3, "seed": 3}
+    cfg["sim_params"]["N"] = 2 ** 20
+    sd = setup_simulation(cfg, tmp_path / "sim", lambda m, r, e: (1e-7, 1e-10), config_dir=tmp_path)
+    assert (sd / "sample_grid.npy").exists()
+    sub = yaml.safe_load((sd / "00000001" / "subconfig.yaml").read_text())
+    assert [m[0] for m, _ in sub["deltabeta_table"]] == ["Si", "Au"]
+    comp = yaml.safe_load((sd / "computed.yaml").read_text())
+    assert len(comp["cutoff_angles"]) == 4
+    assert list(validate_simdir(sd, 1))[:2] == [2 ** 20, 16]

@@ -93,6 +93,50 @@ def test_uniform_sample_equals_uniform_grating():
     assert np.allclose(ref.u, fs.u, rtol=0, atol=1e-12 * np.abs(u).max())
 
 
+def _sample_params(N=2048, dx=2e-9, wl=2.5e-11, chunk=300):
+    return bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=1e-6, detector_pixel_size_x=1e-7,
+                        detector_pixel_size_y=1.0, wl=wl, chunk_size=chunk)
+
+
+def test_single_pixel_sample_is_a_hat():
+    # linear interpolation of one non-vacuum pixel between vacuum neighbours is
+    # the hat max(0, 1 - |x - x_k| / px) times its delta+i*beta; with an even
+    # row length the pixel k = cols/2 sits at x = x_position
+    p = _sample_params()
+    px, pz, db = 6e-8, 1.5e-6, complex(4e-7, 3e-9)
+    for xpos in (0.0, 1.3e-7):
+        grid = np.zeros((1, 8), dtype=np.int8)
+        grid[0, 4] = 1
+        s = bw.Sample(z_start=1e-3, pixel_size_x=px, pixel_size_z=pz, grid=grid,
+                      db_list=np.array([0, db], dtype=np.complex128), x_positions=np.array([xpos]))
+        f = bw.Fields(np.ones(p.N, dtype=np.complex128))
+        bw.sample_apply(f, s, p, 1e30, 0)
+        x = (np.arange(p.N) - p.N / 2) * p.dx
+        hat = np.maximum(0.0, 1.0 - np.abs(x - xpos) / px)
+        ref = bw.Fields(np.exp(2j * np.pi * pz / p.wl * db * hat))
+        bw.propagate(ref, p.dx, p.wl, pz, p.chunk_size, 1e30)
+        assert np.abs(f.u - ref.u).max() < 1e-12
+
+
+def test_sample_x_position_moves_the_sample_to_plus_x():
+    # on a uniform field propagation commutes with shifts, so moving the
+    # sample by s samples (x_position = s*dx) rolls the result by +s
+    p = _sample_params()
+    rng = np.random.default_rng(3)
+    grid = rng.integers(0, 3, size=(2, 30)).astype(np.int8)
+    grid[:, 0] = grid[:, -1] = 0  # vacuum ends: the modulation is continuous
+    dbl = np.array([0, complex(3e-7, 1e-9), complex(8e-7, 6e-9)])
+    out = {}
+    for sft in (0, 37):
+        s = bw.Sample(z_start=1e-3, pixel_size_x=5e-8, pixel_size_z=1e-6, grid=grid, db_list=dbl,
+                      x_positions=np.array([sft * p.dx]))
+        f = bw.Fields(np.ones(p.N, dtype=np.complex128))
+        bw.sample_apply(f, s, p, 1e30, 0)
+        out[sft] = f.u
+    assert np.abs(out[37] - np.roll(out[0], 37)).max() < 1e-9
+    assert np.abs(out[37] - np.roll(out[0], -37)).max() > 1e-3
+
+
 def test_material_factor_attenuates_and_shifts():
     t, wl = 1e-6, 2e-11
     f = bw.material_factor(complex(1e-6, 1e-8), t, wl)
