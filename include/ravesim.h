@@ -74,6 +74,18 @@ typedef struct rs_run_options {
   double *detected;  /* optional caller buffer for the detector rows (n_steps*n_px float64), may be NULL */
   int64_t detected_capacity; /* elements available in `detected` */
   int64_t *shape_out;        /* optional int64[2] receiving (n_steps, n_px) */
+  /* > 0: record big-wave's history with this free-space z spacing
+   * (big-wave/history.py:43-135, multisim.py:130-141): intensity rows every
+   * history_dz between elements (each re-propagated from the segment start,
+   * so the detector rows do not change), after every grating slice and
+   * before every sample row, analytic rows for a point source; written as
+   * %08d/history.npy (float64 (n_px, entries)), history_z.npy, history_x.npy.
+   * 0: none.  fast-wave's `--history_dz` writes the transpose. */
+  double history_dz;
+  /* non-NULL: write the field before and after every element of the first
+   * pass as <dir>/keypoint_%02d_%d.npy in the field precision
+   * (big-wave/wavesim.py:35-58) */
+  const char *keypoints_dir;
 } rs_run_options;
 
 /* Run source `source_idx` of the simulation directory `sim_dir`.

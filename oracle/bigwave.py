@@ -293,8 +293,10 @@ def grating_mask(dx: float, start: int, length: int, g: Grating, z_fraction: flo
     return np.mod(x / g.pitch + dc / 2, 1) < dc
 
 
-def grating_apply(f: Fields, g: Grating, p: SimParams, cutoff_freq: float, step: int) -> None:
-    """Slice loop of a grating plus its substrate; big-wave/optical_element.py:134-206."""
+def grating_apply(f: Fields, g: Grating, p: SimParams, cutoff_freq: float, step: int,
+                  history=None) -> None:
+    """Slice loop of a grating plus its substrate; big-wave/optical_element.py:134-206.
+    With a history, one entry after every slice at z_start + (i + 1) z_step."""
     z_step = g.thickness / g.nr_steps
     fa = material_factor(g.db_a, z_step, p.wl)
     fb = material_factor(g.db_b, z_step, p.wl)
@@ -306,6 +308,9 @@ def grating_apply(f: Fields, g: Grating, p: SimParams, cutoff_freq: float, step:
             fac = np.where(mask, fa, fb).astype(f.u.dtype)
             f.u[idx:idx + length] *= fac
         propagate(f, p.dx, p.wl, z_step, p.chunk_size, cutoff_freq)
+        if history is not None:
+            z = g.z_start + (i + 1) * z_step
+            history.push(square_and_downsample(f.u, p, z), z)
     if g.substrate_thickness > 0:
         f.u *= np.complex128(material_factor(g.db_substrate, g.substrate_thickness, p.wl))
         propagate(f, p.dx, p.wl, g.substrate_thickness, p.chunk_size, cutoff_freq)
@@ -330,7 +335,7 @@ def dba_grating_mask(dx: float, start: int, length: int, g: DBAGrating, z_fracti
 
 
 def dba_grating_apply(f: Fields, g: DBAGrating, p: SimParams, cutoff_freq: float,
-                      step: int) -> None:
+                      step: int, history=None) -> None:
     """Slice loop of a DBA grating plus its substrate; big-wave/optical_element.py:247-316.
     Factor A where the mask holds, B elsewhere (np.place with mask and 1 - mask)."""
     z_step = g.thickness / g.nr_steps
@@ -344,12 +349,16 @@ def dba_grating_apply(f: Fields, g: DBAGrating, p: SimParams, cutoff_freq: float
             fac = np.where(mask, fa, fb).astype(f.u.dtype)
             f.u[idx:idx + length] *= fac
         propagate(f, p.dx, p.wl, z_step, p.chunk_size, cutoff_freq)
+        if history is not None:
+            z = g.z_start + (i + 1) * z_step
+            history.push(square_and_downsample(f.u, p, z), z)
     if g.substrate_thickness > 0:
         f.u *= np.complex128(material_factor(g.db_substrate, g.substrate_thickness, p.wl))
         propagate(f, p.dx, p.wl, g.substrate_thickness, p.chunk_size, cutoff_freq)
 
 
-def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: int) -> None:
+def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: int,
+                 history=None) -> None:
     """Row-by-row sample; big-wave/optical_element.py:432-476.
 
     Each z-row's delta+i*beta is linearly interpolated (np.interp over the
@@ -360,6 +369,9 @@ def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: i
     rowlen = s.grid.shape[1]
     row_x = np.arange(rowlen) * s.pixel_size_x + s.x_positions[step] - rowlen * s.pixel_size_x * 0.5
     for r in range(s.grid.shape[0]):
+        if history is not None:
+            z = s.z_start + r * s.pixel_size_z
+            history.push(square_and_downsample(f.u, p, z), z)
         row_db = s.db_list[s.grid[r, :]]
         for idx in range(0, p.N, p.chunk_size):
             length = min(p.chunk_size, p.N - idx)
@@ -370,13 +382,14 @@ def sample_apply(f: Fields, s: Sample, p: SimParams, cutoff_freq: float, step: i
         propagate(f, p.dx, p.wl, s.pixel_size_z, p.chunk_size, cutoff_freq)
 
 
-def element_apply(f: Fields, el, p: SimParams, cutoff_freq: float, step: int) -> None:
+def element_apply(f: Fields, el, p: SimParams, cutoff_freq: float, step: int,
+                  history=None) -> None:
     if isinstance(el, Grating):
-        grating_apply(f, el, p, cutoff_freq, step)
+        grating_apply(f, el, p, cutoff_freq, step, history)
     elif isinstance(el, DBAGrating):
-        dba_grating_apply(f, el, p, cutoff_freq, step)
+        dba_grating_apply(f, el, p, cutoff_freq, step, history)
     elif isinstance(el, Sample):
-        sample_apply(f, el, p, cutoff_freq, step)
+        sample_apply(f, el, p, cutoff_freq, step, history)
     else:
         raise TypeError(type(el))
 
@@ -386,10 +399,78 @@ def element_apply(f: Fields, el, p: SimParams, cutoff_freq: float, step: int) ->
 # ----------------------------------------------------------------------------
 
 
-def source_propagate_to(src, z_out: float, p: SimParams, cutoff_freq: float, dtype) -> Fields:
+# ----------------------------------------------------------------------------
+# history (big-wave/history.py)
+# ----------------------------------------------------------------------------
+
+
+class History:
+    """Intensity rows with their z values; big-wave/history.py:10-40."""
+
+    def __init__(self, dz: float):
+        self.dz = dz
+        self.entries: list = []
+        self.zs: list = []
+
+    def push(self, downscaled: np.ndarray, z: float) -> None:
+        self.entries.append(downscaled)
+        self.zs.append(z)
+
+    def get_history(self) -> np.ndarray:
+        """(n_px, entries), np.empty((0, 0)) when empty."""
+        if not self.entries:
+            return np.empty((0, 0))
+        return np.stack(self.entries, axis=-1)
+
+
+def generate_analytic_history(history: History, zs: np.ndarray, x_source: float, p: SimParams,
+                              cutoff_freq: float) -> None:
+    """Point-source intensity rows in closed form; big-wave/history.py:43-66."""
+    gradient = cutoff_freq * p.wl
+    n = p.detector_size // p.detector_pixel_size_x
+    x = (np.arange(n) - n // 2) * p.detector_pixel_size_x - x_source
+    for z in zs:
+        angle = np.arctan(x / z)
+        r2 = x ** 2 + z ** 2
+        intensity = p.detector_pixel_size_x * p.detector_pixel_size_y * np.cos(angle) / r2
+        intensity[np.abs(x) > gradient * z] = 0
+        history.push(intensity, z)
+
+
+def propagate_with_history(f: Fields, p: SimParams, dz: float, cutoff_freq: float,
+                           current_z: float, skip_fft: bool = False,
+                           history: Optional[History] = None) -> None:
+    """big-wave/history.py:69-135: entries at zs = arange(0, dz, history.dz)
+    (the last moved to dz when closer than half a step, else dz appended), each
+    propagated from the segment's start spectrum; u and U end as after one
+    propagation over dz."""
+    if history is None:
+        propagate(f, p.dx, p.wl, dz, p.chunk_size, cutoff_freq, skip_fft)
+        return
+    if dz == 0:
+        return
+    zs = np.arange(0, dz, history.dz)
+    if dz - zs[-1] < history.dz / 2.0 and len(zs) > 1:
+        zs[-1] = dz
+    else:
+        zs = np.append(zs, [dz])
+    if not skip_fft:
+        f.U = spfft.fft(f.u)
+    U_start = f.U.copy()
+    for i in range(1, len(zs)):
+        propagate(f, p.dx, p.wl, zs[i] - zs[0], p.chunk_size, cutoff_freq, True)
+        z = zs[i] + current_z
+        history.push(square_and_downsample(f.u, p, z), z)
+        if i < len(zs) - 1:
+            f.U = U_start.copy()
+
+
+def source_propagate_to(src, z_out: float, p: SimParams, cutoff_freq: float, dtype,
+                        history: Optional[History] = None) -> Fields:
     """big-wave/source.py:46-102.  Point source: analytic field then FFT, cutoff,
-    IFFT (U does not hold FFT(u) afterwards).  Vector source: copy when dz == 0,
-    otherwise FFT and propagate with skip_fft."""
+    IFFT (U does not hold FFT(u) afterwards), analytic history rows at
+    arange(0, z_out, dz)[1:].  Vector source: copy when dz == 0, otherwise FFT
+    and propagate with skip_fft."""
     if isinstance(src, PointSource):
         dz = z_out - src.z
         grid_density_check(dz, src.x, p.N, p.dx, p.wl)
@@ -398,13 +479,16 @@ def source_propagate_to(src, z_out: float, p: SimParams, cutoff_freq: float, dty
         f.U = spfft.fft(f.u)
         apply_frequency_cutoff(f.U, cutoff_freq, p.dx, p.chunk_size)
         f.u = spfft.ifft(f.U)
+        if history is not None:
+            zs = np.arange(0, z_out, history.dz)[1:]
+            generate_analytic_history(history, zs, src.x, p, cutoff_freq)
         return f
     dz = z_out - src.z
     f = Fields(src.vec.astype(dtype, copy=True))
     if dz == 0:
         return f
     f.U = spfft.fft(src.vec.astype(dtype))
-    propagate(f, p.dx, p.wl, dz, p.chunk_size, cutoff_freq, skip_fft=True)
+    propagate_with_history(f, p, dz, cutoff_freq, src.z, True, history)
     return f
 
 
@@ -414,34 +498,41 @@ def source_propagate_to(src, z_out: float, p: SimParams, cutoff_freq: float, dty
 
 
 def run_simulation(p: SimParams, source, elements: list, cutoff_angles: Sequence[float],
-                   dtype=np.complex128, final_u: Optional[list] = None) -> list:
+                   dtype=np.complex128, final_u: Optional[list] = None,
+                   history: Optional[History] = None, keypoints: Optional[dict] = None) -> list:
     """One source through all elements incl. phase steps; big-wave/wavesim.py:84-298.
 
     Returns the list of detector rows (one per phase step).  Snapshot before
     the first element with >1 x_positions; the final hop reuses U (skip_fft)
     on the first pass only; elements with one x_position use index 0 in later
     phase steps.  If `final_u` is a list, the final u of every phase step is
-    appended to it.
+    appended to it.  A history records the first pass only; `keypoints`
+    receives u before (element, 0) and after (element, 1) every element of
+    the first pass (wavesim.py:35-58, 182-197).
     """
     fcs = [np.sin(a) / p.wl for a in cutoff_angles]
     fc = fcs[0]
     current_z = elements[0].z_start if elements else p.z_detector
-    f = source_propagate_to(source, current_z, p, fc, dtype)
+    f = source_propagate_to(source, current_z, p, fc, dtype, history)
 
     snapshot = None
     for i, el in enumerate(elements):
         if current_z < el.z_start:
-            propagate(f, p.dx, p.wl, el.z_start - current_z, p.chunk_size, fc)
+            propagate_with_history(f, p, el.z_start - current_z, fc, current_z, False, history)
         current_z = el.z_start
         if len(el.x_positions) > 1 and snapshot is None:
             snapshot = (current_z, i, f.u.copy(), len(el.x_positions))
         fc = fcs[i + 1]
-        element_apply(f, el, p, fc, 0)
+        if keypoints is not None:
+            keypoints[(i, 0)] = f.u.copy()
+        element_apply(f, el, p, fc, 0, history)
+        if keypoints is not None:
+            keypoints[(i, 1)] = f.u.copy()
         current_z = el.z_start + el.get_thickness()
 
     if current_z < p.z_detector - Z_TOLERANCE:
-        propagate(f, p.dx, p.wl, p.z_detector - current_z, p.chunk_size, fc,
-                  skip_fft=len(elements) > 0)
+        propagate_with_history(f, p, p.z_detector - current_z, fc, current_z,
+                               len(elements) > 0, history)
 
     rows = [square_and_downsample(f.u, p, p.z_detector)]
     if final_u is not None:
@@ -572,13 +663,16 @@ def parse_dtype(cfg: dict):
     return {"c8": np.complex64, "c16": np.complex128}[val]
 
 
-def run_simdir(sim_dir, source_idx: int, dtype=None) -> np.ndarray:
+def run_simdir(sim_dir, source_idx: int, dtype=None, history_dz: Optional[float] = None,
+               keypoints: Optional[dict] = None):
     """Oracle for one source of a sim dir; big-wave/multisim.py:50-150.
 
     Reads config.yaml / computed.yaml / %08d/subconfig.yaml (safe YAML loader)
     and returns the float64 (n_steps, n_px) detector array that big-wave saves
     as detected.npy.  `dtype` overrides the config dtype (the oracle is normally
-    run in complex128).
+    run in complex128).  With `history_dz` it returns (detected, history) where
+    history is the History object (multisim.py:130-141 saves
+    history.get_history() and its z values).
     """
     import yaml
 
@@ -597,5 +691,8 @@ def run_simdir(sim_dir, source_idx: int, dtype=None) -> np.ndarray:
         src = PointSource(x=float(s["x"]), z=float(s["z"]))
     else:
         src = VectorSource(vec=np.load(s["input_path"]), z=float(s["z"]))
-    rows = run_simulation(p, src, els, [float(a) for a in comp["cutoff_angles"]], dtype)
-    return np.asarray(rows, dtype=np.float64)
+    hist = History(float(history_dz)) if history_dz is not None else None
+    rows = run_simulation(p, src, els, [float(a) for a in comp["cutoff_angles"]], dtype,
+                          history=hist, keypoints=keypoints)
+    det = np.asarray(rows, dtype=np.float64)
+    return det if hist is None else (det, hist)

@@ -192,7 +192,13 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
       }
       m.upload = src_host.data();
     }
-    const int nsteps = rs::build_member(c, m, 0, save_u);
+    if (!(o.history_dz >= 0)) throw rs::Error(RS_ERR_INVALID, "history_dz must be >= 0");
+    rs::Recording rec;
+    rec.history_dz = o.history_dz;
+    rec.keypoints = o.keypoints_dir != nullptr;
+    std::vector<std::vector<char>> keypoints;
+    if (rec.keypoints) m.keypoints = &keypoints;
+    const int nsteps = rs::build_member(c, m, 0, save_u, &rec);
     rs::RunConfig rc;
     rc.N = c.N;
     rc.dx = c.dx;
@@ -202,26 +208,55 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
     rc.det_px = c.px;
     rc.det_py = c.py;
     rc.z_det = c.z_detector;
-    rc.n_out = nsteps;
+    rc.n_out = rec.n_rows;
+    if (rec.n_rows > nsteps) {
+      // history rows are scaled for their own z
+      rc.row_z.assign((size_t)rec.n_rows, c.z_detector);
+      for (size_t k = 0; k < rec.hist_z.size(); ++k)
+        if (rec.hist_row[k] >= 0) rc.row_z[(size_t)rec.hist_row[k]] = rec.hist_z[k];
+    }
     std::vector<rs::Member> ms;
     ms.push_back(std::move(m));
     std::vector<double> det = rs::run_members(*ctx->ctx, rc, ms);
     const int64_t npx = rs::detector_pixels(c.detector_size, c.px);
+    const size_t ndet = (size_t)nsteps * (size_t)std::max<int64_t>(npx, 0);
     if (o.shape_out) {
       o.shape_out[0] = nsteps;
       o.shape_out[1] = npx;
     }
     if (o.detected) {
-      if (o.detected_capacity < (int64_t)det.size()) throw rs::Error(RS_ERR_INVALID, "detected buffer too small");
-      std::memcpy(o.detected, det.data(), det.size() * sizeof(double));
+      if (o.detected_capacity < (int64_t)ndet) throw rs::Error(RS_ERR_INVALID, "detected buffer too small");
+      std::memcpy(o.detected, det.data(), ndet * sizeof(double));
     }
     if (o.write_output) {
-      rs::npy_save(sub + "/detected.npy", "<f8", {nsteps, npx}, det.data(), det.size() * sizeof(double));
+      rs::npy_save(sub + "/detected.npy", "<f8", {nsteps, npx}, det.data(), ndet * sizeof(double));
       for (size_t s = 0; s < saved.size(); ++s) {
         char name[32];
         std::snprintf(name, sizeof name, "/u_%04d.npy", (int)s);
         rs::npy_save(sub + name, c.f64 ? "<c16" : "<c8", {c.N}, saved[s].data(), saved[s].size());
       }
+    }
+    if (o.write_output && rec.history_dz > 0) {
+      // np.stack(entries, axis=-1): (n_px, entries); np.empty((0, 0)) when none
+      const int64_t H = (int64_t)rec.hist_z.size();
+      std::vector<double> hm((size_t)(H * npx));
+      for (int64_t k = 0; k < H; ++k) {
+        const int r = rec.hist_row[(size_t)k];
+        const double *src = r >= 0 ? det.data() + (size_t)r * npx : rec.analytic[(size_t)k].data();
+        for (int64_t p = 0; p < npx; ++p) hm[(size_t)(p * H + k)] = src[p];
+      }
+      const std::vector<int64_t> shape = H ? std::vector<int64_t>{npx, H} : std::vector<int64_t>{0, 0};
+      rs::npy_save(sub + "/history.npy", "<f8", shape, hm.data(), hm.size() * sizeof(double));
+      rs::npy_save(sub + "/history_z.npy", "<f8", {H}, rec.hist_z.data(), rec.hist_z.size() * sizeof(double));
+      const std::vector<double> hx = rs::detector_x(c.detector_size, c.px);
+      rs::npy_save(sub + "/history_x.npy", "<f8", {(int64_t)hx.size()}, hx.data(), hx.size() * sizeof(double));
+    }
+    for (size_t k = 0; k < keypoints.size(); ++k) {
+      if (keypoints[k].empty()) continue;
+      char name[48];
+      std::snprintf(name, sizeof name, "/keypoint_%02d_%d.npy", (int)(k / 2), (int)(k % 2));
+      rs::npy_save(std::string(o.keypoints_dir) + name, c.f64 ? "<c16" : "<c8", {c.N}, keypoints[k].data(),
+                   keypoints[k].size());
     }
   });
 }

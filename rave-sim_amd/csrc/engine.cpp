@@ -27,7 +27,7 @@ void hip_check(hipError_t e, const char *what) {
 // ----------------------------------------------------------------------------
 
 // CPython's float floor division (Objects/floatobject.c float_floor_div)
-static double py_floordiv(double vx, double wx) {
+double py_floordiv(double vx, double wx) {
   double mod = std::fmod(vx, wx);
   double div = (vx - mod) / wx;
   if (mod != 0.0) {
@@ -430,14 +430,18 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
             "bounds upload");
   const size_t nacc = (size_t)rc.n_out * std::max<int64_t>(npx, 0);
 
-  bool needs_snap = false;
+  bool needs_snap = false, needs_hist = false;
   for (auto &m : members)
-    for (auto &o : m.ops) needs_snap |= o.kind == OpKind::Restore;
+    for (auto &o : m.ops) {
+      needs_snap |= o.kind == OpKind::Restore;
+      needs_hist |= o.kind == OpKind::HSave;
+    }
+  const size_t per_member = 1 + (needs_snap ? 1 : 0) + (needs_hist ? 1 : 0);  // buffers
 
   const size_t fbytes = (size_t)rc.N * 2 * sizeof(S);
   int group = rc.max_resident > 0 ? rc.max_resident : (int)members.size();
   if (rc.max_resident <= 0 && !rc.fields)
-    group = default_group(fbytes * (needs_snap ? 2 : 1), (int64_t)members.size());
+    group = default_group(fbytes * per_member, (int64_t)members.size());
   if (rc.fields) group = (int)members.size();
   group = std::max(1, std::min(group, 65535));
 
@@ -468,13 +472,15 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   // of the lane: each member program starts by generating or uploading its
   // field.  All groups are compiled first, their argument records go to the
   // device in one copy, and the launches are enqueued back to back.
-  std::vector<std::vector<DevBuf>> fbuf(lanes), sbuf(lanes);
+  std::vector<std::vector<DevBuf>> fbuf(lanes), sbuf(lanes), hbuf(lanes);
   const size_t width = std::min(members.size(), (size_t)group);
   for (int l = 0; l < lanes; ++l) {
     if (!rc.fields)
       for (size_t i = 0; i < width; ++i) fbuf[l].emplace_back(fbytes);
     if (needs_snap)
       for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
+    if (needs_hist)
+      for (size_t i = 0; i < width; ++i) hbuf[l].emplace_back(fbytes);
   }
   // sample elements: device copies of the members' pixel tables for this run,
   // pointed at by their SAMPLE masks
@@ -517,6 +523,7 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
       ex->mem.push_back(&members[i]);
       ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);
       if (needs_snap) ex->snap.push_back(sbuf[lane][i - g0].p);
+      if (needs_hist) ex->hist.push_back(hbuf[lane][i - g0].p);
     }
     ex->acc = (double *)dacc[lane].p;
     ex->bounds = (const int64_t *)dbounds.p;
@@ -531,7 +538,7 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   // last-level cache anyway: when they fit, the extra stream evicts them and
   // costs more than the decisions it saves.
   MaskCache<S> masks;
-  const size_t resident = (size_t)lanes * width * fbytes * (needs_snap ? 2 : 1);
+  const size_t resident = (size_t)lanes * width * fbytes * per_member;
   if (rc.mode == 0 && resident > kMallBytes)
     for (auto &ex : groups) masks.collect(*ex);
   masks.fill(rc.N, ctx.stream);
@@ -584,14 +591,18 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   for (hipEvent_t e : evs) (void)hipEventDestroy(e);
   for (auto &ex : groups) ex->collect_timing();
 
-  const std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
+  if (!rc.row_z.empty() && (int)rc.row_z.size() != rc.n_out) throw Error(1, "row_z must have n_out entries");
+  std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
   std::vector<double> out(nacc);
-  for (int r = 0; r < rc.n_out; ++r)
+  for (int r = 0; r < rc.n_out; ++r) {
+    if (!rc.row_z.empty() && (r == 0 || rc.row_z[(size_t)r] != rc.row_z[(size_t)r - 1]))
+      sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.row_z[(size_t)r]);
     for (int64_t k = 0; k < npx; ++k) {
       double v = accs[0][(size_t)r * npx + k];
       for (int l = 1; l < lanes; ++l) v += accs[l][(size_t)r * npx + k];
       out[(size_t)r * npx + k] = (((v * sc[2 * k]) * rc.dx) * rc.det_py) / sc[2 * k + 1];
     }
+  }
   return out;
 }
 

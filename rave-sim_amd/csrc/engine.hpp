@@ -39,6 +39,9 @@ void hip_check(hipError_t e, const char *what);
 // [rs, re) zeroed by the cutoff, big-wave/propagation.py:83-90; (0,0) if none
 void cutoff_range(int64_t N, double fc, double dx, int64_t *rs, int64_t *re);
 
+// Python's float floor division vx // wx
+double py_floordiv(double vx, double wx);
+
 // number of detector pixels, int(size // px) (big-wave/propagation.py:286)
 int64_t detector_pixels(double size, double px);
 
@@ -92,14 +95,18 @@ std::vector<std::pair<int, int>> plan_candidates(int lgN, bool f64);
 // member programs
 // ----------------------------------------------------------------------------
 
-enum class OpKind { Source, Upload, Mask, Prop, Detect, Snapshot, Restore, SaveU, Flush };
+// Snapshot / Restore: the phase-stepping restart point.  HSave / HRestore: a
+// second saved state for history entries (big-wave/history.py:74-135
+// re-propagates every entry from the segment's start).  SaveU / Keypoint:
+// natural-order field downloads (u_%04d.npy, keypoint_%02d_%d.npy).
+enum class OpKind { Source, Upload, Mask, Prop, Detect, Snapshot, Restore, SaveU, Flush, HSave, HRestore, Keypoint };
 
 struct Op {
   OpKind kind;
   XOp x{};          // Source (POINT / ONES) and Mask (MASK / SCALAR / SAMPLE)
   double dz = 0;    // Prop
   double fc = 0;    // Prop cutoff frequency (big-wave semantics)
-  int row = 0;      // Detect: output row; SaveU: phase step
+  int row = 0;      // Detect: output row; SaveU: phase step; Keypoint: 2 element + (0 before, 1 after)
   int sample = -1;  // SAMPLE masks: index into Member::samples and the grid row
   int sample_row = 0;
 };
@@ -120,6 +127,7 @@ struct Member {
   std::vector<SampleRows> samples;  // device copies are made per run
   const void *upload = nullptr; // host data for the Upload op (natural order)
   std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
+  std::vector<std::vector<char>> *keypoints = nullptr;  // Keypoint sink (indexed by the op's row)
 };
 
 struct RunConfig {
@@ -129,6 +137,7 @@ struct RunConfig {
   int mode = 0;              // 0 fused, 1 plain
   double det_size = 0, det_px = 0, det_py = 1, z_det = 0;
   int n_out = 1;             // accumulator rows
+  std::vector<double> row_z; // detector distance of each row (empty: z_det for all)
   int max_resident = 0;      // members per group; 0 = as many as fit the last-level cache together
   bool use_graph = false;
   int lanes = 0;             // streams groups alternate over; 0 = 2 with the default grouping

@@ -30,6 +30,7 @@ struct Launch {
   int64_t n = 0;
   int member = 0;
   int step = 0;
+  bool keypoint = false;  // SaveU: into the member's keypoint sink
 };
 
 // pending state of one lockstep group
@@ -153,6 +154,7 @@ template <typename S> struct Exec {
   std::vector<Member *> mem;          // this group
   std::vector<void *> field;          // per member
   std::vector<void *> snap;           // per member (phase stepping)
+  std::vector<void *> hist;           // per member (history entries)
   std::vector<FieldArgs> fargs;
   std::vector<DetArgs> dargs;
   std::vector<Launch> launches;
@@ -299,6 +301,7 @@ template <typename S> struct Exec {
     bool transformed = false;
     const bool plain = rc.mode == 1;
     bool snap_transformed = false;
+    bool hist_transformed = false;
 
     auto clear_px = [&]() {
       for (auto &x : px) std::memset(&x, 0, sizeof(XOp));
@@ -383,34 +386,41 @@ template <typename S> struct Exec {
         case OpKind::Flush:
           flush_natural();
           break;
-        case OpKind::Snapshot: {
+        case OpKind::Snapshot:
+        case OpKind::HSave: {
+          // the saved state is the in-flight one (natural or column-transformed)
           bool any = false;
           for (auto &x : px) any |= x.kind != XOP_NONE;
           if (any) flush_natural();
-          if (!snap.empty()) {
-            for (int i = 0; i < K; ++i) {
-              Launch l{LKind::Copy};
-              l.dst = snap[i];
-              l.src = field[i];
-              l.n = rc.N;
-              launches.push_back(l);
-            }
+          const bool h = kind == OpKind::HSave;
+          const std::vector<void *> &to = h ? hist : snap;
+          if (h && to.empty()) throw Error(1, "history buffer missing");
+          for (size_t i = 0; i < to.size(); ++i) {
+            Launch l{LKind::Copy};
+            l.dst = to[i];
+            l.src = field[i];
+            l.n = rc.N;
+            launches.push_back(l);
           }
-          snap_transformed = transformed;
+          (h ? hist_transformed : snap_transformed) = transformed;
           break;
         }
         case OpKind::Restore:
+        case OpKind::HRestore: {
+          const bool h = kind == OpKind::HRestore;
           for (int i = 0; i < K; ++i) {
             Launch l{LKind::Copy};
             l.dst = field[i];
-            l.src = snap[i];
+            l.src = h ? hist[i] : snap[i];
             l.n = rc.N;
             launches.push_back(l);
           }
           clear_px();
-          transformed = snap_transformed;
+          transformed = h ? hist_transformed : snap_transformed;
           break;
-        case OpKind::SaveU: {
+        }
+        case OpKind::SaveU:
+        case OpKind::Keypoint: {
           flush_natural();
           for (int i = 0; i < K; ++i) {
             Launch l{LKind::SaveU};
@@ -418,6 +428,7 @@ template <typename S> struct Exec {
             l.n = rc.N;
             l.member = i;
             l.step = mem[i]->ops[o].row;
+            l.keypoint = kind == OpKind::Keypoint;
             launches.push_back(l);
           }
           break;
@@ -532,12 +543,13 @@ template <typename S> struct Exec {
         case LKind::SaveU: {
           end_capture();
           Member *m = mem[l.member];
-          if (m->saved_u) {
+          std::vector<std::vector<char>> *sink = l.keypoint ? m->keypoints : m->saved_u;
+          if (sink) {
             std::vector<char> host(l.n * esz);
             hip_check(hipMemcpyAsync(host.data(), l.src, l.n * esz, hipMemcpyDeviceToHost, st), "download u");
             hip_check(hipStreamSynchronize(st), "sync");
-            if ((int)m->saved_u->size() <= l.step) m->saved_u->resize(l.step + 1);
-            (*m->saved_u)[l.step] = std::move(host);
+            if ((int)sink->size() <= l.step) sink->resize(l.step + 1);
+            (*sink)[l.step] = std::move(host);
           }
           break;
         }

@@ -18,7 +18,7 @@ static void usage(const char *argv0) {
       "Usage: %s <sim_dir> -s <source_idx> [options]\n"
       "  <sim_dir>              Main simulation directory\n"
       "  -s, --source_idx <i>   The index of the source point to simulate (required)\n"
-      "  --history_dz <f>       Z-Spacing for history entries (not supported by this build)\n"
+      "  --history_dz <f>       Z-Spacing for history entries (big-wave's history layout: (n_px, entries))\n"
       "  --dtype c8|c16         Override the precision given in config.yaml\n"
       "  --plain                Run one kernel per operation instead of fused passes\n"
       "  --device <n>           HIP device index (default 0)\n"
@@ -31,7 +31,7 @@ int main(int argc, char **argv) {
   int source_idx = -1;
   bool have_src = false;
   int dtype = 0, mode = RS_MODE_FUSED, device = 0;
-  bool history = false;
+  double history_dz = 0;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto need = [&](const char *what) -> const char * {
@@ -52,8 +52,11 @@ int main(int argc, char **argv) {
       source_idx = std::atoi(a.c_str() + 13);
       have_src = true;
     } else if (a == "--history_dz") {
-      (void)need("history_dz");
-      history = true;
+      history_dz = std::atof(need("history_dz"));
+      if (!(history_dz > 0)) {
+        std::fprintf(stderr, "history_dz must be positive\n");
+        return 1;
+      }
     } else if (a == "--dtype") {
       const std::string d = need("dtype");
       if (d == "c8") dtype = RS_C8;
@@ -88,10 +91,6 @@ int main(int argc, char **argv) {
     usage(argv[0]);
     return 1;
   }
-  if (history) {
-    std::fprintf(stderr, "uncaught exception in the fast-wave framework: --history_dz is not supported by this build\n");
-    return 1;
-  }
   rs_ctx *ctx = nullptr;
   if (rs_ctx_create(device, &ctx) != RS_OK) {
     std::fprintf(stderr, "uncaught exception in the fast-wave framework: %s\n", rs_last_error());
@@ -101,6 +100,7 @@ int main(int argc, char **argv) {
   o.dtype = dtype;
   o.mode = mode;
   o.write_output = 1;
+  o.history_dz = history_dz;
   const int rc = rs_run_simdir(ctx, sim_dir.c_str(), source_idx, &o);
   if (rc != RS_OK) std::fprintf(stderr, "uncaught exception in the fast-wave framework: %s\n", rs_last_error());
   rs_ctx_destroy(ctx);

@@ -246,7 +246,8 @@ XOp mask_op(const SimConfig &c, const GratingCfg &g, int slice, int step, const 
 
 }  // namespace
 
-void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops) {
+void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops,
+                 const SliceHook &hook) {
   const double zs = g.thickness / (double)g.nr_steps;
   double fa[2], fb[2];
   material_factor(g.db_a.real(), g.db_a.imag(), zs, c.wl, fa);
@@ -259,6 +260,7 @@ void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, s
     p.dz = zs;
     p.fc = fc;
     ops.push_back(p);
+    if (hook) hook(g.z_start + (double)(i + 1) * zs);
   }
   if (g.substrate_thickness > 0) {
     double fs[2];
@@ -277,7 +279,7 @@ void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, s
   }
 }
 
-void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m) {
+void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m, const SliceHook &hook) {
   // row_x = np.arange(rowlen) * px + x_positions[step] - rowlen * px * 0.5
   SampleRows sr;
   sr.len = g.cols;
@@ -302,6 +304,7 @@ void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Me
   m.samples.push_back(std::move(sr));
   const double a = (2.0 * M_PI * g.pixel_size_z) / c.wl;  // as material_factor forms it
   for (int64_t r = 0; r < g.rows; ++r) {
+    if (hook) hook(g.z_start + (double)r * g.pixel_size_z);
     Op mk{OpKind::Mask};
     mk.x.kind = XOP_SAMPLE;
     mk.x.post[0] = 1.0;
@@ -333,7 +336,41 @@ void grid_density_check(double dz, double xs, int64_t N, double dx, double wl) {
 
 }  // namespace
 
-int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
+std::vector<double> detector_x(double size, double px) {
+  const double n = py_floordiv(size, px);
+  const double h = py_floordiv(n, 2.0);
+  std::vector<double> x((size_t)std::max<int64_t>((int64_t)n, 0));
+  for (size_t k = 0; k < x.size(); ++k) x[k] = ((double)k - h) * px;
+  return x;
+}
+
+namespace {
+
+// Analytic history rows of a point source (big-wave/history.py:43-66) at
+// z = i * hdz for 0 < i < ceil(z_out / hdz) (source.py:65-68).
+void analytic_history(const SimConfig &c, double z_out, double hdz, double fc, Recording &rec) {
+  const double gradient = fc * c.wl;
+  const double n = py_floordiv(c.detector_size, c.px);
+  const double h = py_floordiv(n, 2.0);
+  const int64_t cnt = (int64_t)std::ceil(z_out / hdz);
+  for (int64_t i = 1; i < cnt; ++i) {
+    const double z = (double)i * hdz;
+    std::vector<double> row((size_t)std::max<int64_t>((int64_t)n, 0));
+    for (size_t k = 0; k < row.size(); ++k) {
+      const double x = ((double)k - h) * c.px - c.source.x;
+      const double ang = std::atan(x / z);
+      const double r2 = x * x + z * z;
+      row[k] = std::fabs(x) > gradient * z ? 0.0 : ((c.px * c.py) * std::cos(ang)) / r2;
+    }
+    rec.hist_z.push_back(z);
+    rec.hist_row.push_back(-1);
+    rec.analytic.push_back(std::move(row));
+  }
+}
+
+}  // namespace
+
+int build_member(const SimConfig &c, Member &m, int out_base, bool save_u, Recording *rec) {
   check_config(c);
   m.wl = c.wl;
   m.out_base = out_base;
@@ -343,6 +380,54 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
   std::vector<double> fcs;
   for (double a : c.cutoff_angles) fcs.push_back(std::sin(a) / c.wl);
   const double tol = 1e-8;
+
+  size_t nsteps = 1;
+  for (const auto &g : c.elements) nsteps = std::max(nsteps, g.x_positions.size());
+  const bool hist = rec && rec->history_dz > 0;
+  if (rec) {
+    rec->hist_z.clear();
+    rec->hist_row.clear();
+    rec->analytic.clear();
+    rec->n_rows = (int)nsteps;
+  }
+  // one history entry read from the device into the next free row
+  auto hist_detect = [&](double z) {
+    Op d{OpKind::Detect};
+    d.row = rec->n_rows++;
+    ops.push_back(d);
+    rec->hist_z.push_back(z);
+    rec->hist_row.push_back(d.row);
+  };
+  const SliceHook hook = hist ? SliceHook(hist_detect) : SliceHook();
+  // big-wave/history.py:69-135 (propagate_with_history): entries at
+  // zs = np.arange(0, dz, hdz) with the last one moved or appended to dz,
+  // each propagated from the segment's start state
+  auto propagate = [&](double dz, double z0, double fc_) {
+    if (!hist) {
+      Op p{OpKind::Prop};
+      p.dz = dz;
+      p.fc = fc_;
+      ops.push_back(p);
+      return;
+    }
+    if (dz == 0) return;
+    const double hdz = rec->history_dz;
+    const double cnt = std::ceil(dz / hdz);
+    if (!(cnt <= 1e6)) throw Error(2, "history_dz gives more than 10^6 entries in one segment");
+    std::vector<double> zs((size_t)cnt);
+    for (size_t i = 0; i < zs.size(); ++i) zs[i] = (double)i * hdz;
+    if (dz - zs.back() < hdz / 2.0 && zs.size() > 1) zs.back() = dz;
+    else zs.push_back(dz);
+    if (zs.size() > 2) ops.push_back(Op{OpKind::HSave});
+    for (size_t i = 1; i < zs.size(); ++i) {
+      Op p{OpKind::Prop};
+      p.dz = zs[i] - zs[0];
+      p.fc = fc_;
+      ops.push_back(p);
+      hist_detect(zs[i] + z0);
+      if (i + 1 < zs.size()) ops.push_back(Op{OpKind::HRestore});
+    }
+  };
 
   double fc = fcs[0];
   double cur = c.elements.empty() ? c.z_detector : c.elements[0].z_start;
@@ -362,6 +447,7 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
     p.dz = 0.0;
     p.fc = fc;
     ops.push_back(p);
+    if (hist) analytic_history(c, cur, rec->history_dz, fc, *rec);
   } else {
     if (c.source.kind == SourceCfg::Plane) {
       Op s{OpKind::Source};
@@ -372,46 +458,48 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
       ops.push_back(Op{OpKind::Upload});
     }
     const double dz = cur - c.source.z;
-    if (dz != 0) {
-      Op p{OpKind::Prop};
-      p.dz = dz;
-      p.fc = fc;
-      ops.push_back(p);
-    }
+    if (dz != 0) propagate(dz, c.source.z, fc);
   }
 
   int snap_idx = -1;
   double snap_z = 0;
-  size_t nsteps = 1;
   for (size_t i = 0; i < c.elements.size(); ++i) {
     const GratingCfg &g = c.elements[i];
-    if (cur < g.z_start) {
-      Op p{OpKind::Prop};
-      p.dz = g.z_start - cur;
-      p.fc = fc;
-      ops.push_back(p);
-    }
+    if (cur < g.z_start) propagate(g.z_start - cur, cur, fc);
     cur = g.z_start;
     if (g.x_positions.size() > 1 && snap_idx < 0) {
       snap_idx = (int)i;
       snap_z = cur;
-      nsteps = g.x_positions.size();
       ops.push_back(Op{OpKind::Snapshot});
     }
     fc = fcs[i + 1];
-    if (g.sample) sample_ops(c, g, 0, fc, m);
-    else grating_ops(c, g, 0, fc, ops);
+    if (rec && rec->keypoints) {
+      Op k{OpKind::Keypoint};
+      k.row = 2 * (int)i;
+      ops.push_back(k);
+    }
+    if (g.sample) sample_ops(c, g, 0, fc, m, hook);
+    else grating_ops(c, g, 0, fc, ops, hook);
+    if (rec && rec->keypoints) {
+      Op k{OpKind::Keypoint};
+      k.row = 2 * (int)i + 1;
+      ops.push_back(k);
+    }
     cur = g.z_start + g.total_thickness();
   }
-  auto final_hop = [&]() {
+  auto final_hop = [&](bool first) {
     if (cur < c.z_detector - tol) {
-      Op p{OpKind::Prop};
-      p.dz = c.z_detector - cur;
-      p.fc = fc;
-      ops.push_back(p);
+      if (first) {
+        propagate(c.z_detector - cur, cur, fc);
+      } else {
+        Op p{OpKind::Prop};
+        p.dz = c.z_detector - cur;
+        p.fc = fc;
+        ops.push_back(p);
+      }
     }
   };
-  final_hop();
+  final_hop(true);
   {
     Op d{OpKind::Detect};
     d.row = 0;
@@ -442,7 +530,7 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u) {
         else grating_ops(c, g, st, fc, ops);
         cur = g.z_start + g.total_thickness();
       }
-      final_hop();
+      final_hop(false);
       Op d{OpKind::Detect};
       d.row = (int)step;
       ops.push_back(d);

@@ -4,6 +4,7 @@
 // references differ, big-wave's semantics are followed.
 #pragma once
 #include <complex>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -59,18 +60,46 @@ std::string subdir_name(int source_idx);  // "%08d"
 // (fast-wave/src/config_parsing.cpp:281-307, big-wave/multisim.py:70-112).
 SimConfig parse_simdir(const std::string &sim_dir, int source_idx);
 
+// Optional recordings of the first pass through the elements (phase step 0).
+struct Recording {
+  // History (big-wave/history.py, wavesim.py:153-219): with history_dz > 0,
+  // intensity rows every history_dz in free space (re-propagated from the
+  // segment's start), after every grating slice and before every sample row;
+  // a point source contributes analytic rows up to the first element.
+  double history_dz = 0;
+  // Keypoints (big-wave/wavesim.py:35-58): the field before (sub 0) and
+  // after (sub 1) every element, as Keypoint ops with row 2 * element + sub.
+  bool keypoints = false;
+  // filled by build_member: history entries in push order with their z, the
+  // accumulator row of each device entry (-1: the analytic row at the same
+  // index of `analytic`), and the accumulator rows used in total
+  std::vector<double> hist_z;
+  std::vector<int> hist_row;
+  std::vector<std::vector<double>> analytic;
+  int n_rows = 0;
+};
+
 // Build the member program of one source, incl. phase steps.  Returns the
-// number of detector rows (phase steps).
-int build_member(const SimConfig &c, Member &m, int out_base, bool save_u);
+// number of detector rows (phase steps); history rows follow them.
+int build_member(const SimConfig &c, Member &m, int out_base, bool save_u, Recording *rec = nullptr);
+
+// Called with z after every grating slice and before every sample row
+// (history entries of big-wave/optical_element.py:182-187, 448-453).
+using SliceHook = std::function<void(double z)>;
 
 // Slice loop + substrate of one grating as member ops, using
 // x_positions[step] (big-wave/optical_element.py:134-206).
-void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops);
+void grating_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, std::vector<Op> &ops,
+                 const SliceHook &hook = nullptr);
 
 // Row loop of a sample element as member ops (one SAMPLE mask and one
 // propagation over pixel_size_z per grid row), its pixel tables appended to
 // m.samples (big-wave/optical_element.py:432-476).
-void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m);
+void sample_ops(const SimConfig &c, const GratingCfg &g, int step, double fc, Member &m,
+                const SliceHook &hook = nullptr);
+
+// detector pixel x positions (k - n // 2) * px (big-wave/util.py:78-82)
+std::vector<double> detector_x(double size, double px);
 
 // Check element ordering / phase-step consistency like big-wave's
 // check_simulation_inputs (big-wave/multisim.py:195-242); throws Error.

@@ -47,7 +47,8 @@ class RsError(RuntimeError):
 class rs_run_options(C.Structure):
     _fields_ = [("dtype", C.c_int), ("mode", C.c_int), ("write_output", C.c_int),
                 ("reserved", C.c_int), ("detected", C.POINTER(C.c_double)),
-                ("detected_capacity", C.c_int64), ("shape_out", C.POINTER(C.c_int64))]
+                ("detected_capacity", C.c_int64), ("shape_out", C.POINTER(C.c_int64)),
+                ("history_dz", C.c_double), ("keypoints_dir", C.c_char_p)]
 
 
 class rs_grid(C.Structure):

@@ -40,11 +40,12 @@ def run_single_simulation(sim_dir, source_idx: int, scratch_dir=None, save_keypo
 
     dtype: None (take config.yaml's, default c16) or 'c8' / 'c16' to force it.
     plain: one kernel per operation instead of fused passes (baseline form).
+    history_dz: record big-wave's history into history.npy / history_z.npy /
+    history_x.npy of the source's directory (big-wave/multisim.py:130-141).
+    save_keypoints_path: write keypoint_%02d_%d.npy there (big-wave/wavesim.py:35-58).
     """
-    if save_keypoints_path is not None:
-        raise NotImplementedError("keypoint saving is not supported by this build")
-    if history_dz is not None:
-        raise NotImplementedError("history recording is not supported by this build")
+    if history_dz is not None and not float(history_dz) > 0:
+        raise ValueError("history_dz must be positive")
     start = time.perf_counter()
     sim_dir = Path(sim_dir)
     info = validate_simdir(sim_dir, source_idx)
@@ -55,7 +56,9 @@ def run_single_simulation(sim_dir, source_idx: int, scratch_dir=None, save_keypo
         dtype={None: 0, "c8": _lib.RS_C8, "c16": _lib.RS_C16}[dtype],
         mode=_lib.RS_MODE_PLAIN if plain else _lib.RS_MODE_FUSED, write_output=1, reserved=0,
         detected=out.ctypes.data_as(C.POINTER(C.c_double)), detected_capacity=out.size,
-        shape_out=C.cast(shape, C.POINTER(C.c_int64)))
+        shape_out=C.cast(shape, C.POINTER(C.c_int64)),
+        history_dz=float(history_dz) if history_dz is not None else 0.0,
+        keypoints_dir=str(save_keypoints_path).encode() if save_keypoints_path is not None else None)
     check(lib.rs_run_simdir((ctx or _ctx()).handle, str(sim_dir).encode(), int(source_idx), C.byref(opts)))
     logger.info(f"Finished running simulation for source {source_idx} in {time.perf_counter() - start} seconds")
     return out

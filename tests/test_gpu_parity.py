@@ -364,3 +364,51 @@ def test_dba_grating_simdir_matches_oracle(ctx, tmp_path, dtype, tol, chunk):
     print(f"DBA grating {dtype} chunk {chunk} rel-L2 = {err:.3e}")
     assert err < tol
     assert rel_l2(got[0], got[1]) > 1e-3  # the two grating positions differ
+
+
+@pytest.mark.parametrize("case,dtype,tol", [("sample", "c16", 1e-10), ("sample", "c8", 1e-4),
+                                            ("dba", "c16", 1e-10), ("vector", "c16", 1e-10)])
+def test_history_and_keypoints_match_oracle(ctx, tmp_path, case, dtype, tol):
+    """history_dz and save_keypoints_path of run_single_simulation: the
+    detector rows are unchanged, history.npy is (n_px, entries) with the
+    oracle's z values exactly, keypoint files hold the field before and after
+    every element of the first pass."""
+    import shutil
+    from ravesim.multisim import run_single_simulation
+    from ravesim.simdir import write_simdir
+    from simcases import dba_case, sample_case
+    if case == "sample":
+        cfg, subs, ang = sample_case(tmp_path, dtype=dtype)
+    else:
+        cfg, subs, ang = dba_case(dtype=dtype)
+    if case == "vector":  # a plane wave from 4 mm before the first element
+        N = cfg["sim_params"]["N"]
+        np.save(tmp_path / "ones.npy", np.ones(N, dtype=np.complex128))
+        subs[0]["source"] = {"type": "vector", "input_path": str(tmp_path / "ones.npy"), "z": 0.006}
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    if case == "sample":
+        shutil.copy(tmp_path / "sample_grid.npy", sd / "sample_grid.npy")
+    hdz = 0.0037
+    kdir = tmp_path / "kp"
+    kdir.mkdir()
+    got = run_single_simulation(sd, 0, tmp_path, save_keypoints_path=kdir, history_dz=hdz, ctx=ctx)
+    kp = {}
+    ref, h = bw.run_simdir(sd, 0, dtype=np.complex128, history_dz=hdz, keypoints=kp)
+    assert rel_l2(got, ref) < tol
+    sub = sd / "00000000"
+    hist, hz, hx = (np.load(sub / f) for f in ("history.npy", "history_z.npy", "history_x.npy"))
+    href = h.get_history()
+    print(f"history {case} {dtype}: {hist.shape} rel-L2 {rel_l2(hist, href):.3e}")
+    assert hist.shape == href.shape and hist.shape[1] > 8
+    assert np.array_equal(hz, np.asarray(h.zs, dtype=np.float64))
+    assert np.array_equal(hx, bw.detector_x_vector(cfg["sim_params"]["detector_size"],
+                                                   cfg["sim_params"]["detector_pixel_size_x"]))
+    assert rel_l2(hist, href) < tol
+    for k in range(hist.shape[1]):
+        if np.linalg.norm(href[:, k]) > 0:
+            assert rel_l2(hist[:, k], href[:, k]) < 10 * tol, f"entry {k} z {hz[k]}"
+    names = sorted(p.name for p in kdir.iterdir())
+    assert names == sorted(f"keypoint_{i:02}_{s}.npy" for i, s in kp)
+    ftol = 1e-10 if dtype == "c16" else 2e-4
+    for (i, s), u in kp.items():
+        assert rel_l2(np.load(kdir / f"keypoint_{i:02}_{s}.npy"), u) < ftol, f"keypoint {i} {s}"
