@@ -664,7 +664,7 @@ def parse_dtype(cfg: dict):
 
 
 def run_simdir(sim_dir, source_idx: int, dtype=None, history_dz: Optional[float] = None,
-               keypoints: Optional[dict] = None):
+               keypoints: Optional[dict] = None, final_u: Optional[list] = None):
     """Oracle for one source of a sim dir; big-wave/multisim.py:50-150.
 
     Reads config.yaml / computed.yaml / %08d/subconfig.yaml (safe YAML loader)
@@ -693,6 +693,6 @@ def run_simdir(sim_dir, source_idx: int, dtype=None, history_dz: Optional[float]
         src = VectorSource(vec=np.load(s["input_path"]), z=float(s["z"]))
     hist = History(float(history_dz)) if history_dz is not None else None
     rows = run_simulation(p, src, els, [float(a) for a in comp["cutoff_angles"]], dtype,
-                          history=hist, keypoints=keypoints)
+                          final_u=final_u, history=hist, keypoints=keypoints)
     det = np.asarray(rows, dtype=np.float64)
     return det if hist is None else (det, hist)

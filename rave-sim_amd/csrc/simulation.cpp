@@ -170,8 +170,10 @@ void check_config(const SimConfig &c) {
   if (c.chunk_size <= 0) throw Error(2, "chunk_size must be positive");
   if (!(c.px > 0) || !(c.detector_size > 0)) throw Error(2, "Detector size and pixel size must be positive");
   if (detector_pixels(c.detector_size, c.px) > c.N) throw Error(2, "detector has more pixels than samples");
-  if (c.cutoff_angles.size() != c.elements.size() + 1)
-    throw Error(2, "cutoff_angles must have len(elements) + 1 entries");
+  // a sim dir set up with a save_and_exit element keeps the angles of the
+  // elements cut after it (big-wave/multisim.py:166-192, 499-507)
+  if (c.cutoff_angles.size() < c.elements.size() + 1)
+    throw Error(2, "cutoff_angles must have at least len(elements) + 1 entries");
   const double tol = 1e-8;
   double cur = c.source.z;
   size_t nsteps = 0;
