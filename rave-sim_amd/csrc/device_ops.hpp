@@ -429,21 +429,29 @@ __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 
   return mul ? cmul(v, f) : f;
 }
 
-// Grating mask decision at sample n (single chunk, factors folded on the
-// host): true selects fa.  The fast test t = fma(n, m_a, m_b) is within
-// m_guard of big-wave's t = x/pitch + dc/2; a fraction within m_guard of an
-// edge (0, dc or 1) is decided on the exact FP64 path instead.
-__device__ __forceinline__ bool mask_in(const XOp &op, double one_minus_guard, int32_t n) {
+// Grating decisions of one thread's 16 samples n0 + m*ms (single chunk,
+// factors folded on the host); in[m] selects fa.  t_m = fma(m, ms*m_a, t_0)
+// with t_0 = fma(n0, m_a, m_b) is within m_guard of big-wave's x/pitch + dc/2
+// (two roundings of at most an ulp of the span each, far inside the guard);
+// a fraction within m_guard of an edge (0, dc or 1) is decided on the exact
+// FP64 path, sample by sample (ties recur every few thousand samples when
+// pitch/dx is rational).
+__device__ __forceinline__ void mask_tile(const XOp &op, int32_t n0, int32_t ms, bool (&in)[16]) {
 #pragma clang fp contract(off)
-  const double nd = (double)n;
-  const double t = __builtin_fma(nd, op.m_a, op.m_b);
-  double ph = t - floor(t);
-  if (ph < op.m_guard || ph > one_minus_guard || fabs(ph - op.dc) < op.m_guard) {
-    const double x = nd * op.xstep + op.x0;
-    const double te = x / op.pitch + op.half_dc;
-    ph = te - floor(te);
+  const double t0 = __builtin_fma((double)n0, op.m_a, op.m_b);
+  const double d = (double)ms * op.m_a;
+  const double omg = 1.0 - op.m_guard;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const double t = __builtin_fma((double)m, d, t0);
+    double ph = t - floor(t);
+    if (ph < op.m_guard || ph > omg || fabs(ph - op.dc) < op.m_guard) {
+      const double x = (double)(n0 + m * ms) * op.xstep + op.x0;
+      const double te = x / op.pitch + op.half_dc;
+      ph = te - floor(te);
+    }
+    in[m] = ph < op.dc;
   }
-  return ph < op.dc;
 }
 
 // ---------------------------------------------------------------------------
@@ -491,10 +499,9 @@ struct KopF32 {
   double au, bu;         // alpha, beta in 1/kCisTab revolutions
   float s1, s3, c2, c4, sc;  // sc*w, -sc*w^3/6, -sc*w^2/2, sc*w^4/24, sc (w = phi per unit of f)
   int rs, rlen;          // cutoff band [rs, rs + rlen)
-  int shl;               // 32 - lgN: k' = (k << shl) >> shl
 };
 
-__device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab, int lgN) {
+__device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab) {
   KopF32 p;
   p.tab = tab;
   p.au = op.alpha_rev * (double)kCisTab;
@@ -508,14 +515,32 @@ __device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab
   p.c4 = (float)(sc * w * w * w * w / 24.0);
   p.rs = (int)op.rs;
   p.rlen = (int)(op.re - op.rs);
-  p.shl = 32 - lgN;
   return p;
 }
 
-template <bool CUT> __device__ __forceinline__ float2 kop_f32(const KopF32 &p, int k, float2 v) {
-  const int ks = (int)((uint32_t)k << p.shl) >> p.shl;
-  const double kd = (double)ks;
-  const double u = __builtin_fma(p.au, kd * kd, p.bu);
+// Row-pass spectrum factors of one thread: its 16 bins are k = k0 + m N/16
+// with 0 <= k0 < N/16, so the signed bins are k0 + m' N/16 (m' = m for m < 8,
+// m - 16 above) and u = U0 + U1 m' + U2 m'^2 with U0 = 256 (alpha k0^2 + beta),
+// U1 = 512 alpha k0 N/16, U2 = 256 alpha (N/16)^2, evaluated by Horner's rule
+// in FP64 (a few ulp of the largest term, far below complex64 rounding).
+struct KopQuad {
+  double u0, u1, u2;
+};
+__device__ __forceinline__ KopQuad kop_f32_quad(const KopF32 &p, int k0, int64_t N) {
+  const double kd = (double)k0;
+  const double D = (double)(N >> 4);
+  KopQuad q;
+  q.u0 = __builtin_fma(p.au, kd * kd, p.bu);
+  q.u1 = ((2.0 * p.au) * kd) * D;
+  q.u2 = (p.au * D) * D;
+  return q;
+}
+__device__ __forceinline__ double kop_f32_u(const KopQuad &q, int m) {
+  const double mm = (double)(m < 8 ? m : m - 16);
+  return __builtin_fma(mm, __builtin_fma(mm, q.u2, q.u1), q.u0);
+}
+
+template <bool CUT> __device__ __forceinline__ float2 kop_f32(const KopF32 &p, double u, int k, float2 v) {
   const double j = rint(u);
   const float f = (float)(u - j);
   // j mod kCisTab from the low mantissa bits of j + 1.5 * 2^52 (|j| < 2^51)

@@ -142,25 +142,20 @@ __global__ void __launch_bounds__(C *(L / 16))
         T2 fa, fb;
         fa.x = (S)x.fa[0]; fa.y = (S)x.fa[1];
         fb.x = (S)x.fb[0]; fb.y = (S)x.fb[1];
+        // a generated plane wave holds ones here, so the products are the
+        // factors themselves
         if (x.kind == XOP_MASK && !ROW && x.mbits) {
           // decisions cached for this tile shape (mask_fill_kernel)
 #pragma unroll
-          for (int m = 0; m < 16; ++m) {
-            const T2 f = ((mword >> m) & 1u) ? fa : fb;
-            v[m] = gen ? f : cmul(v[m], f);
-          }
+          for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], ((mword >> m) & 1u) ? fa : fb);
         } else if (x.kind == XOP_MASK) {
-          const double omg = 1.0 - x.m_guard;
-          const int32_t n0 = (int32_t)(nb + lane);
-          const int32_t ms = (int32_t)mst;
+          bool in[16];
+          mask_tile(x, (int32_t)(nb + lane), (int32_t)mst, in);
 #pragma unroll
-          for (int m = 0; m < 16; ++m) {
-            const T2 f = mask_in(x, omg, n0 + m * ms) ? fa : fb;
-            v[m] = gen ? f : cmul(v[m], f);
-          }
+          for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], in[m] ? fa : fb);
         } else {
 #pragma unroll
-          for (int m = 0; m < 16; ++m) v[m] = gen ? fa : cmul(v[m], fa);
+          for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], fa);
         }
       }
     } else if constexpr (XV == XV_GEN) {
@@ -186,15 +181,16 @@ __global__ void __launch_bounds__(C *(L / 16))
       const int64_t kb = row_kbase(g, rr);
       const int64_t kstep = g.N >> g.lgR;
       if constexpr (sizeof(S) == 4) {
-        const KopF32 kp = kop_f32_setup(A.k, ktab, 63 - __clzll((unsigned long long)g.N));
-        const int k0 = (int)(kb + kstep * q);
-        const int kst = (int)(kstep * TPF);
+        const KopF32 kp = kop_f32_setup(A.k, ktab);
+        const int k0 = (int)(kb + kstep * q);  // < N/16
+        const int kst = (int)(kstep * TPF);    // = N/16
+        const KopQuad kq = kop_f32_quad(kp, k0, g.N);
         if (kp.rlen != 0) {
 #pragma unroll
-          for (int m = 0; m < 16; ++m) v[m] = kop_f32<true>(kp, k0 + m * kst, v[m]);
+          for (int m = 0; m < 16; ++m) v[m] = kop_f32<true>(kp, kop_f32_u(kq, m), k0 + m * kst, v[m]);
         } else {
 #pragma unroll
-          for (int m = 0; m < 16; ++m) v[m] = kop_f32<false>(kp, k0 + m * kst, v[m]);
+          for (int m = 0; m < 16; ++m) v[m] = kop_f32<false>(kp, kop_f32_u(kq, m), k0 + m * kst, v[m]);
         }
       } else {
 #pragma unroll
@@ -221,12 +217,11 @@ __global__ void __launch_bounds__(C *(L / 16))
 template <int L, int C>
 __global__ void __launch_bounds__(C *(L / 16)) mask_fill_kernel(XOp x, Geom g, uint16_t *__restrict__ bits) {
   const TileMap<L, C, false> tm(g, blockIdx.x, threadIdx.x);
-  const double omg = 1.0 - x.m_guard;
-  const int32_t n0 = (int32_t)(tm.nb + tm.lane);
-  const int32_t ms = (int32_t)tm.mst;
+  bool in[16];
+  mask_tile(x, (int32_t)(tm.nb + tm.lane), (int32_t)tm.mst, in);
   uint32_t w = 0;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) w |= (mask_in(x, omg, n0 + m * ms) ? 1u : 0u) << m;
+  for (int m = 0; m < 16; ++m) w |= (in[m] ? 1u : 0u) << m;
   bits[(int64_t)blockIdx.x * (C * (L / 16)) + threadIdx.x] = (uint16_t)w;
 }
 
