@@ -99,6 +99,141 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
             "plain_s_by_split": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in plain_runs.items()}}
 
 
+def uneven_mix_sims(ravesim, n_sims=256, seed=11, slices=32):
+    """The uneven mix: n_sims simulations, N = 2^k with k uniform on 16..24
+    (log-uniform sizes), energy uniform on 20-70 keV, grating period uniform
+    on 2-6 um with duty cycle 0.4-0.6 (the single-grating chain: Si, 30 um,
+    `slices` slices, delta ~ E^-2 and beta ~ E^-4 from the 25 keV values,
+    0.1 m hop), 1-16 point sources 5 m before the grating at lateral offsets
+    uniform within +-0.5 um, their intensities summed incoherently.
+    Returns [(lgN, [sources of sim])]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    h = 30e-6
+    sims = []
+    for _ in range(n_sims):
+        lgN = int(rng.integers(16, 25))
+        e = float(rng.uniform(20.0, 70.0))
+        pitch = float(rng.uniform(2e-6, 6e-6))
+        dc = float(rng.uniform(0.4, 0.6))
+        nsrc = int(rng.integers(1, 17))
+        xs = rng.uniform(-0.5e-6, 0.5e-6, nsrc)
+        sc = 25.0 / e
+        g = ravesim.Grating(pitch=pitch, dc=(dc, dc), z_start=0.0, thickness=h, nr_steps=slices,
+                            db_a=complex(7.7e-7 * sc ** 2, 3.5e-10 * sc ** 4))
+        wl = ravesim.energy_to_wavelength(e * 1000.0)
+        sims.append((lgN, [ravesim.core.BatchMember(wl=wl, gratings=[g], cutoff_angles=[math.pi / 2] * 2,
+                                                    source_x=float(x), source_z=-5.0) for x in xs]))
+    return sims, h + 0.1, slices + 1
+
+
+def uneven_mix(ravesim, ctx, n_sims=256):
+    """Fused complex64 batches (one lockstep batch per grid size, split tuned
+    per size and batch size, each simulation's sources summed into its own
+    detector row) vs the plain complex128 form one simulation at a time."""
+    import numpy as np
+    from ravesim.core import run_batch
+    sims, zdet, steps = uneven_mix_sims(ravesim, n_sims)
+    dx = 2e-9
+    by_n = {}
+    for i, (lgN, srcs) in enumerate(sims):
+        by_n.setdefault(lgN, []).append(i)
+
+    def kw(lgN):
+        return dict(det_size=(2 ** lgN) * dx / 4, det_px=1e-6, det_py=1.0, z_detector=zdet)
+
+    def fused_run():
+        out = {}
+        for lgN, idx in sorted(by_n.items()):
+            ms = []
+            for r, i in enumerate(idx):
+                for m in sims[i][1]:
+                    m.out_index = r
+                    ms.append(m)
+            rows = run_batch(ctx, 2 ** lgN, dx, ms, n_outputs=len(idx), dtype=np.complex64, **kw(lgN))
+            for r, i in enumerate(idx):
+                out[i] = rows[r]
+        return out
+
+    t0 = time.perf_counter()
+    for lgN, idx in sorted(by_n.items()):
+        ctx.autotune(2 ** lgN, np.complex64, members=sum(len(sims[i][1]) for i in idx))
+    tune_s = time.perf_counter() - t0
+    fused_run()  # warm: plans, tables
+    t0 = time.perf_counter()
+    fused = fused_run()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    err_num = err_den = 0.0
+    for i, (lgN, srcs) in enumerate(sims):
+        for m in srcs:
+            m.out_index = 0
+        row = run_batch(ctx, 2 ** lgN, dx, srcs, dtype=np.complex128, plain=True, max_resident=1,
+                        **kw(lgN))[0]
+        err_num += float(np.sum((fused[i] - row) ** 2))
+        err_den += float(np.sum(row ** 2))
+    tp = time.perf_counter() - t0
+    n_src = sum(len(s) for _, s in sims)
+    samples = sum(len(s) * 2 ** lgN for lgN, s in sims)
+    return {"workload": f"{n_sims} sims, N = 2^16..2^24 log-uniform, 1-16 point sources each "
+                        f"({n_src} in all), {steps} steps per source, E 20-70 keV, p 2-6 um, dc 0.4-0.6",
+            "fused_c64_batched_sims_per_hour": round(n_sims / tf * 3600, 1),
+            "fused_gsample_steps_per_s": round(samples * steps / tf / 1e9, 2),
+            "plain_c128_one_at_a_time_sims_per_hour": round(n_sims / tp * 3600, 1),
+            "speedup": round(tp / tf, 2), "rel_l2_fused_vs_plain_c128": math.sqrt(err_num / err_den),
+            "fused_s": round(tf, 4), "plain_s": round(tp, 4), "tune_s": round(tune_s, 2),
+            "sims_per_size": {f"2^{k}": len(v) for k, v in sorted(by_n.items())}}
+
+
+def rocfft_comparison_child(N):
+    """rocfft_comparison in a child process: torch carries its own HIP runtime,
+    which cannot share a process with the one this library loaded."""
+    import subprocess
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--vendor-fft-only", str(N)],
+                       capture_output=True, text=True, timeout=240)
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}: {r.stderr.strip()[-300:]}"}
+    return json.loads(lines[-1])
+
+
+def rocfft_comparison(N=2 ** 26, reps=20):
+    """One propagation step FFT * propagator * iFFT on N complex64 samples with
+    the vendor FFT (torch.fft -> hipFFT -> rocFFT), as a reference point for
+    this library's fused propagation step."""
+    import torch
+    if not torch.cuda.is_available():
+        return None
+    dev = torch.device("cuda", 0)
+    u = torch.ones(N, dtype=torch.complex64, device=dev)
+    f = torch.fft.fftfreq(N, d=2e-9, device=dev, dtype=torch.float64)
+    wl, dz = 2.695e-11, 1e-6
+    ph = (math.pi * wl * dz) * f * f - 2 * math.pi * dz / wl
+    H = torch.polar(torch.ones_like(ph), torch.remainder(ph, 2 * math.pi)).to(torch.complex64)
+    del f, ph
+    for _ in range(3):
+        u = torch.fft.ifft(torch.fft.fft(u) * H)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        u = torch.fft.ifft(torch.fft.fft(u) * H)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    e0.record()
+    for _ in range(reps):
+        v = torch.fft.fft(u)
+    e1.record()
+    torch.cuda.synchronize()
+    fft_ms = e0.elapsed_time(e1) / reps
+    del u, H, v
+    torch.cuda.empty_cache()
+    return {"what": f"torch.fft (hipFFT/rocFFT) fft * H * ifft, N=2^{int(math.log2(N))} complex64",
+            "ms_per_propagation_step": round(ms, 4), "ms_per_fft": round(fft_ms, 4),
+            "fft_GBs_at_16B_per_sample": round(16 * N / (fft_ms * 1e-3) / 1e9, 1)}
+
+
 def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_energies=4):
     """Poisson arrivals of Talbot-Lau phase-stepping requests (8 G2 positions,
     3 x 64 slices) at a fixed rate; N log-uniform over `sizes`, 1..max_energies
@@ -125,6 +260,15 @@ def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_e
     res = [srv.result(i, timeout_s=600) for i, _ in ids]
     wall = time.perf_counter() - t0
     st = srv.stats()
+    # error vs FP64 per request: the smallest requests served again in complex128
+    errs = []
+    for k in sorted(range(len(arrivals)), key=lambda k: arrivals[k][1].N)[:3]:
+        r = arrivals[k][1]
+        ref = srv.result(srv.submit(phase_sweep(r.N, r.energies_ev, n_steps=8, slices=64,
+                                                dtype=np.complex128)), timeout_s=600).rows
+        errs.append({"request": k, "N": r.N, "energies": len(r.energies_ev),
+                     "rel_l2": float(np.linalg.norm(res[k].rows - ref) / np.linalg.norm(ref)),
+                     "fringe_phase_err_rad": fringe_phase_error(res[k].rows, ref)})
     srv.close()
     lat = np.array([r.latency_s for r in res])
     sims = sum(ne for _, ne in ids)
@@ -133,7 +277,22 @@ def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_e
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "sims_per_hour": round(sims / wall * 3600, 1),
             "mix": f"N in 2^{list(sizes)}, 1-{max_energies} energies, 8 phase steps, 650 steps per energy",
-            "jobs_per_launch": round(st["jobs_per_launch"], 2), "max_resident_jobs": st["max_resident_jobs"]}
+            "jobs_per_launch": round(st["jobs_per_launch"], 2), "max_resident_jobs": st["max_resident_jobs"],
+            "error_vs_fp64": errs}
+
+
+def fringe_phase_error(rows, ref):
+    """Largest fringe-phase difference (rad) of a phase-stepping scan: per
+    pixel -atan2(sum_k I_k sin(2 pi k / n), sum_k I_k cos(2 pi k / n)), over the
+    pixels whose first-harmonic amplitude exceeds 1e-3 of the mean intensity."""
+    import numpy as np
+    n = rows.shape[0]
+    w = np.exp(2j * np.pi * np.arange(n) / n)[:, None]
+    a, b = (w * rows).sum(0), (w * ref).sum(0)
+    ok = np.abs(b) > 1e-3 * np.abs(ref).mean() * n
+    if not ok.any():
+        return None
+    return float(np.max(np.abs(np.angle(a[ok] * np.conj(b[ok])))))
 
 
 def cpu_baseline(N, budget_s=25.0):
@@ -174,16 +333,26 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
     ap.add_argument("--split", default="", help="fixed digit-group split 'lgR,lgA' instead of tuning")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparisons")
+    ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
+    ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
     ap.add_argument("--serve-qps", type=float, default=1.0,
                     help="serving run arrival rate in requests/s (0 = skip); the mix saturates near 1.7/s")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-sizes", default="22,24,26",
+                    help="log2 grid sizes of served requests (the full mix: 22,26,28,30)")
+    ap.add_argument("--serve-max-energies", type=int, default=4, help="energies per request: 1..this (full: 32)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
+    ap.add_argument("--vendor-fft-only", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.vendor_fft_only:
+        print(json.dumps(rocfft_comparison(args.vendor_fft_only)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every configuration of this path runs on a single "
-              "MI355X (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "MI355X, and separate simulations scale out as one process per GPU; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -238,9 +407,23 @@ def main():
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
     batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx)
+    mix = None
+    if not (args.no_batch or args.plain) and args.mix_sims > 0:
+        mix = uneven_mix(ravesim, ctx, args.mix_sims)
+    ours_step_ms = ms_per_step / steps_per_sim
+    vendor = None
+    if not args.no_rocfft and not args.plain:
+        try:
+            vendor = rocfft_comparison_child(N)
+        except Exception as e:  # a comparison only: report why it is missing
+            vendor = {"error": repr(e)}
+        if vendor and "ms_per_propagation_step" in vendor:
+            vendor["this_library_ms_per_step_incl_grating"] = round(ours_step_ms, 4)
     serving = None
     if args.serve_qps > 0 and not args.plain:
-        serving = serving_run(ravesim, ctx, args.serve_qps, args.serve_seconds)
+        serving = serving_run(ravesim, ctx, args.serve_qps, args.serve_seconds,
+                              sizes=tuple(int(v) for v in args.serve_sizes.split(",")),
+                              max_energies=args.serve_max_energies)
 
     cpu = None
     if not args.no_cpu:
@@ -262,6 +445,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched_vs_plain": batch,
+        "uneven_mix": mix,
+        "vendor_fft": vendor,
         "serving": serving,
         "detector_checksum": float(np.sum(out)),
     }

@@ -5,7 +5,7 @@ big-wave's sim-dir schema.  Full-size and reduced-size variants share the
 physics (energies, gratings, distances); reduced variants shrink N, slices and
 the detector so that the complex128 oracle finishes in seconds.
 
-Builder decisions (BASELINE leaves them open): chunk_size = N unless given,
+Choices the workload descriptions leave open: chunk_size = N unless given,
 dx = 2 nm, plane waves are a vector source of ones at the first grating,
 no frequency cutoff (all cutoff angles pi/2) for plane-wave chains, the
 detector follows the last element directly when the config says so.
@@ -17,7 +17,7 @@ from pathlib import Path
 
 import numpy as np
 
-# delta/beta of the configurations
+# delta/beta of the monochromatic chains (Si at 25 keV, Au and Si at 46 keV)
 DB_SI_25KEV = (7.7e-7, 3.5e-10)
 DB_AU_46KEV = (1.1e-6, 4.0e-8)
 DB_SI_46KEV = (2.6e-7, 1.0e-10)
