@@ -452,3 +452,25 @@ def test_save_and_exit_then_continue_matches_oracle(ctx, tmp_path):
     print(f"continuation rel-L2 {rel_l2(got, ref):.3e}")
     assert got.shape == ref.shape == (1, 10)
     assert rel_l2(got, ref) < 1e-10
+
+
+def test_fastwave_cli_matches_oracle(tmp_path):
+    """The fastwave executable on a sim dir (phase-stepped DBA + grating
+    chain), with --history_dz: detected.npy and history.npy against the
+    oracle, files in big-wave's layout."""
+    import subprocess
+    from pathlib import Path
+    from ravesim.simdir import write_simdir
+    from simcases import dba_case
+    exe = Path(__file__).resolve().parent.parent / "rave-sim_amd" / "ravesim" / "fastwave"
+    cfg, subs, ang = dba_case(dtype="c16")
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    r = subprocess.run([str(exe), str(sd), "-s", "0", "--history_dz", "0.0037"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.load(sd / "00000000" / "detected.npy")
+    ref, h = bw.run_simdir(sd, 0, dtype=np.complex128, history_dz=0.0037)
+    assert got.dtype == np.float64 and got.shape == ref.shape
+    assert rel_l2(got, ref) < 1e-10
+    hist = np.load(sd / "00000000" / "history.npy")
+    assert hist.shape == h.get_history().shape and rel_l2(hist, h.get_history()) < 1e-10

@@ -253,3 +253,22 @@ def test_setup_simulation_with_sample_and_dba(tmp_path):
     comp = yaml.safe_load((sd / "computed.yaml").read_text())
     assert len(comp["cutoff_angles"]) == 4
     assert list(validate_simdir(sd, 1))[:2] == [2 ** 20, 16]
+
+
+FASTWAVE = Path(__file__).resolve().parent.parent / "rave-sim_amd" / "ravesim" / "fastwave"
+
+
+@pytest.mark.skipif(not FASTWAVE.exists(), reason="fastwave is not built")
+def test_fastwave_cli_arguments():
+    """fast-wave's command line (fast-wave/main.cpp:25-57): -h prints the help
+    and exits 0; a missing source index or sim dir is an error with exit 1."""
+    import subprocess
+    r = subprocess.run([str(FASTWAVE), "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--source_idx" in r.stdout and "--history_dz" in r.stdout
+    r = subprocess.run([str(FASTWAVE), "/nonexistent"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Source index is required" in r.stderr
+    r = subprocess.run([str(FASTWAVE), "-s", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Simulation directory is required" in r.stderr
+    r = subprocess.run([str(FASTWAVE), "x", "-s", "0", "--history_dz", "-1"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "history_dz" in r.stderr
