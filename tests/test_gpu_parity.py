@@ -474,3 +474,49 @@ def test_fastwave_cli_matches_oracle(tmp_path):
     assert rel_l2(got, ref) < 1e-10
     hist = np.load(sd / "00000000" / "history.npy")
     assert hist.shape == h.get_history().shape and rel_l2(hist, h.get_history()) < 1e-10
+
+
+def test_cfg2_full_size_precision_form(ctx, tmp_path):
+    """The precision form at the benchmark's full size: Talbot-Lau chain at
+    N = 2^26 with G2 stepped over 8 positions (16 slices per grating).  Beyond
+    the oracle's reach the reference is the plain complex128 GPU form (itself
+    checked against the oracle at smaller N); the complex64 + FP64-phase form
+    stays within 1e-4 relative L2 of intensity and 1e-3 rad of fringe phase."""
+    from ravesim.multisim import run_single_simulation
+    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 26, slices=16, det_size=1e-3, dtype="c8", g2_steps=8))
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = run_single_simulation(sd, 0, tmp_path, ctx=ctx, dtype="c16", plain=True)
+    assert got.shape == ref.shape == (8, 1000)
+    err = rel_l2(got, ref)
+    k = np.arange(8)
+    w = np.exp(2j * np.pi * k / 8)[:, None]
+    a, b = (w * got).sum(0), (w * ref).sum(0)
+    ok = np.abs(b) > 1e-3 * np.abs(ref).mean() * 8
+    dphi = np.abs(np.angle(a[ok] * np.conj(b[ok]))).max()
+    print(f"cfg2 N=2^26 c8 vs plain c16: rel-L2 {err:.3e}, fringe phase {dphi:.2e} rad over {ok.sum()} px")
+    assert err < 1e-4 and dphi < 1e-3
+
+
+def test_n30_field_round_trip_and_composition(ctx):
+    """A 2^30-sample complex64 field (8 GiB, the largest served size): a point
+    source generated on the device, propagated forward and back, returns to
+    its detector image; two hops equal one."""
+    import ravesim
+    N, dx, wl = 2 ** 30, 2e-9, 2.695e-11
+    size, px = 1e-3, 1e-6
+    a = ravesim.Field(ctx, N, dx, dtype=np.complex64)
+    a.point_source(wl, 1e-7, 200.0)  # Nyquist at 2^30 samples needs z > 160 m
+    d0 = a.square_and_downsample(size, px, 1.0, 200.0)
+    a.propagate(wl, 0.05)
+    a.propagate(wl, 0.07)
+    d1 = a.square_and_downsample(size, px, 1.0, 200.12)
+    a.propagate(wl, -0.12)
+    d2 = a.square_and_downsample(size, px, 1.0, 200.0)
+    a.close()
+    b = ravesim.Field(ctx, N, dx, dtype=np.complex64)
+    b.point_source(wl, 1e-7, 200.0)
+    b.propagate(wl, 0.12)
+    d3 = b.square_and_downsample(size, px, 1.0, 200.12)
+    b.close()
+    print(f"2^30: round trip {rel_l2(d2, d0):.2e}, two hops vs one {rel_l2(d1, d3):.2e}")
+    assert np.all(d0 > 0) and rel_l2(d2, d0) < 1e-5 and rel_l2(d1, d3) < 1e-5
