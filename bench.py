@@ -252,12 +252,18 @@ def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_e
     srv = Server(ctx)
     t0 = time.perf_counter()
     ids = []
-    for at, req in arrivals:
+    for k, (at, req) in enumerate(arrivals):
         dt = at - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
         ids.append((srv.submit(req), len(req.energies_ev)))
-    res = [srv.result(i, timeout_s=600) for i, _ in ids]
+        print(f"[serve] t={time.perf_counter() - t0:.1f}s submitted {k}: N=2^{int(math.log2(req.N))}, "
+              f"{len(req.energies_ev)} energies", file=sys.stderr, flush=True)
+    res = []
+    for k, (i, _) in enumerate(ids):
+        res.append(srv.result(i, timeout_s=900))
+        print(f"[serve] t={time.perf_counter() - t0:.1f}s request {k} done, latency {res[-1].latency_s:.2f}s",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     st = srv.stats()
     # error vs FP64 per request: the smallest requests served again in complex128
@@ -342,6 +348,7 @@ def main():
     ap.add_argument("--serve-sizes", default="22,24,26",
                     help="log2 grid sizes of served requests (the full mix: 22,26,28,30)")
     ap.add_argument("--serve-max-energies", type=int, default=4, help="energies per request: 1..this (full: 32)")
+    ap.add_argument("--serve-seed", type=int, default=7, help="seed of the arrival process")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
     ap.add_argument("--vendor-fft-only", type=int, default=0, help=argparse.SUPPRESS)
@@ -423,7 +430,7 @@ def main():
     if args.serve_qps > 0 and not args.plain:
         serving = serving_run(ravesim, ctx, args.serve_qps, args.serve_seconds,
                               sizes=tuple(int(v) for v in args.serve_sizes.split(",")),
-                              max_energies=args.serve_max_energies)
+                              max_energies=args.serve_max_energies, seed=args.serve_seed)
 
     cpu = None
     if not args.no_cpu:
