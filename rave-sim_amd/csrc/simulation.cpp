@@ -399,6 +399,7 @@ int build_member(const SimConfig &c, Member &m, int out_base, bool save_u, Recor
     ops.push_back(d);
     rec->hist_z.push_back(z);
     rec->hist_row.push_back(d.row);
+    rec->analytic.emplace_back();  // one slot per entry: device rows leave it empty
   };
   const SliceHook hook = hist ? SliceHook(hist_detect) : SliceHook();
   // big-wave/history.py:69-135 (propagate_with_history): entries at

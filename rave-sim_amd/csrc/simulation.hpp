@@ -70,9 +70,10 @@ struct Recording {
   // Keypoints (big-wave/wavesim.py:35-58): the field before (sub 0) and
   // after (sub 1) every element, as Keypoint ops with row 2 * element + sub.
   bool keypoints = false;
-  // filled by build_member: history entries in push order with their z, the
-  // accumulator row of each device entry (-1: the analytic row at the same
-  // index of `analytic`), and the accumulator rows used in total
+  // filled by build_member, one element per history entry in push order: its
+  // z, its accumulator row (-1 for an analytic entry) and, for an analytic
+  // entry, its intensity row (empty otherwise); n_rows counts the
+  // accumulator rows used in total
   std::vector<double> hist_z;
   std::vector<int> hist_row;
   std::vector<std::vector<double>> analytic;
