@@ -146,6 +146,19 @@ typedef struct rs_dba_grating {
   double db_s_re, db_s_im;
 } rs_dba_grating;
 
+/* Sample element: a grid of material indices, row r (z) and column k (x),
+ * 0 = vacuum, i = deltabeta[i - 1] (big-wave/optical_element.py:380-430). */
+typedef struct rs_sample {
+  double z_start;
+  double pixel_size_x, pixel_size_z;
+  int64_t rows, cols;
+  const int8_t *grid;        /* rows x cols, row-major */
+  int32_t nr_materials;
+  int32_t nr_x_positions;
+  const double *deltabeta;   /* nr_materials x (delta, beta) */
+  const double *x_positions;
+} rs_sample;
+
 typedef struct rs_detector {
   double size;       /* detector_size [m] */
   double pixel_x;    /* detector_pixel_size_x */
@@ -186,6 +199,13 @@ int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step
  * x_positions[phase_step]. */
 int rs_apply_dba_grating(rs_field *f, double wl, const rs_dba_grating *g, int phase_step,
                          double cutoff_freq);
+
+/* Sample rows: per grid row, u *= exp(2 pi i pixel_size_z / wl * db(x)) with
+ * db the np.interp of delta + i beta between the pixel positions
+ * k px + x_positions[phase_step] - cols px / 2 (zero outside), then a
+ * propagation over pixel_size_z (big-wave/optical_element.py:432-476;
+ * fast-wave/fwcuda/kernels.cu:118-150 `apply_sample_factors`). */
+int rs_apply_sample(rs_field *f, double wl, const rs_sample *s, int phase_step, double cutoff_freq);
 
 /* u *= factor (fast-wave/fwcuda/kernels.cu:160-172). */
 int rs_scale(rs_field *f, double re, double im);

@@ -404,6 +404,37 @@ int rs_apply_dba_grating(rs_field *f, double wl, const rs_dba_grating *g, int ph
   });
 }
 
+int rs_apply_sample(rs_field *f, double wl, const rs_sample *s, int phase_step, double cutoff_freq) {
+  if (!f || !s || !s->grid || !s->x_positions || (s->nr_materials > 0 && !s->deltabeta))
+    return fail(RS_ERR_INVALID, "NULL argument");
+  if (s->rows < 1 || s->cols < 1) return fail(RS_ERR_INVALID, "sample grid must not be empty");
+  if (!(s->pixel_size_x > 0) || !(s->pixel_size_z > 0))
+    return fail(RS_ERR_INVALID, "sample pixel sizes must be positive");
+  if (s->nr_materials < 0 || s->nr_x_positions < 1) return fail(RS_ERR_INVALID, "bad material or position count");
+  if (phase_step < 0 || phase_step >= s->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  return guarded([&] {
+    rs::SimConfig c = field_cfg(f, wl);
+    rs::GratingCfg g;
+    g.sample = true;
+    g.z_start = s->z_start;
+    g.pixel_size_x = s->pixel_size_x;
+    g.pixel_size_z = s->pixel_size_z;
+    g.rows = s->rows;
+    g.cols = s->cols;
+    g.grid.assign(s->grid, s->grid + s->rows * s->cols);
+    for (int8_t v : g.grid)
+      if (v < 0 || v > s->nr_materials) throw rs::Error(RS_ERR_INVALID, "sample grid index out of range");
+    g.db_list.assign(1, {0, 0});
+    for (int i = 0; i < s->nr_materials; ++i) g.db_list.push_back({s->deltabeta[2 * i], s->deltabeta[2 * i + 1]});
+    g.x_positions.assign(s->x_positions, s->x_positions + s->nr_x_positions);
+    rs::Member m;
+    m.wl = wl;
+    rs::sample_ops(c, g, phase_step, cutoff_freq, m);
+    m.ops.push_back(rs::Op{rs::OpKind::Flush});
+    run_on_field(f, m, field_rc(f));
+  });
+}
+
 int rs_scale(rs_field *f, double re, double im) {
   if (!f) return fail(RS_ERR_INVALID, "field is NULL");
   return guarded([&] {

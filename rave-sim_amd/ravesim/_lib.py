@@ -31,7 +31,7 @@ EXPORTS = (
     "rs_version", "rs_last_error", "rs_ctx_create", "rs_ctx_destroy", "rs_ctx_synchronize",
     "rs_run_simdir", "rs_validate_simdir", "rs_field_create", "rs_field_destroy",
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
-    "rs_apply_grating", "rs_apply_dba_grating", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
+    "rs_apply_grating", "rs_apply_dba_grating", "rs_apply_sample", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
     "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
     "rs_server_destroy",
@@ -73,6 +73,13 @@ class rs_dba_grating(C.Structure):
                 ("db_a_re", C.c_double), ("db_a_im", C.c_double),
                 ("db_b_re", C.c_double), ("db_b_im", C.c_double),
                 ("db_s_re", C.c_double), ("db_s_im", C.c_double)]
+
+
+class rs_sample(C.Structure):
+    _fields_ = [("z_start", C.c_double), ("pixel_size_x", C.c_double), ("pixel_size_z", C.c_double),
+                ("rows", C.c_int64), ("cols", C.c_int64), ("grid", C.POINTER(C.c_int8)),
+                ("nr_materials", C.c_int32), ("nr_x_positions", C.c_int32),
+                ("deltabeta", C.POINTER(C.c_double)), ("x_positions", C.POINTER(C.c_double))]
 
 
 class rs_detector(C.Structure):
@@ -129,6 +136,7 @@ def _load() -> C.CDLL:
         "rs_apply_grating": (C.c_int, [P, C.c_double, C.POINTER(rs_grating), C.c_int, C.c_double]),
         "rs_apply_dba_grating": (C.c_int, [P, C.c_double, C.POINTER(rs_dba_grating), C.c_int,
                                            C.c_double]),
+        "rs_apply_sample": (C.c_int, [P, C.c_double, C.POINTER(rs_sample), C.c_int, C.c_double]),
         "rs_scale": (C.c_int, [P, C.c_double, C.c_double]),
         "rs_square_and_downsample": (C.c_int, [P, C.POINTER(rs_detector), C.c_double,
                                                C.POINTER(C.c_double), C.c_int64]),

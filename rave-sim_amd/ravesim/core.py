@@ -150,6 +150,37 @@ class DBAGrating:
         return self.thickness + self.substrate_thickness
 
 
+@dataclass
+class Sample:
+    """Sample element, fields as in big-wave/optical_element.py:380-408 with
+    the materials' delta + i beta resolved: grid[r, k] = 0 is vacuum, i > 0
+    selects deltabeta[i - 1]."""
+
+    z_start: float
+    pixel_size_x: float
+    pixel_size_z: float
+    grid: np.ndarray
+    deltabeta: Sequence[complex]
+    x_positions: Sequence[float] = (0.0,)
+
+    def to_c(self):
+        g = np.ascontiguousarray(self.grid, dtype=np.int8)
+        if g.ndim != 2:
+            raise ValueError("sample grid must be two-dimensional")
+        db = np.array([[complex(v).real, complex(v).imag] for v in self.deltabeta], dtype=np.float64)
+        db = np.ascontiguousarray(db.reshape(-1))
+        xs = np.ascontiguousarray(np.asarray(self.x_positions, dtype=np.float64))
+        s = _lib.rs_sample(z_start=self.z_start, pixel_size_x=self.pixel_size_x,
+                           pixel_size_z=self.pixel_size_z, rows=g.shape[0], cols=g.shape[1],
+                           grid=g.ctypes.data_as(C.POINTER(C.c_int8)), nr_materials=len(self.deltabeta),
+                           nr_x_positions=len(xs), deltabeta=db.ctypes.data_as(C.POINTER(C.c_double)),
+                           x_positions=xs.ctypes.data_as(C.POINTER(C.c_double)))
+        return s, (g, db, xs)
+
+    def get_thickness(self) -> float:
+        return self.pixel_size_z * np.asarray(self.grid).shape[0]
+
+
 class Field:
     """A device-resident complex field (rs_field) with big-wave's operators."""
 
@@ -209,6 +240,12 @@ class Field:
                           cutoff_freq: float = math.inf) -> None:
         cg, keep = g.to_c()
         check(lib.rs_apply_dba_grating(self._h, wl, C.byref(cg), phase_step, cutoff_freq))
+        del keep
+
+    def apply_sample(self, wl: float, s: Sample, phase_step: int = 0,
+                     cutoff_freq: float = math.inf) -> None:
+        cs, keep = s.to_c()
+        check(lib.rs_apply_sample(self._h, wl, C.byref(cs), phase_step, cutoff_freq))
         del keep
 
     def scale(self, factor: complex) -> None:

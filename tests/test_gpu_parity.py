@@ -520,3 +520,31 @@ def test_n30_field_round_trip_and_composition(ctx):
     b.close()
     print(f"2^30: round trip {rel_l2(d2, d0):.2e}, two hops vs one {rel_l2(d1, d3):.2e}")
     assert np.all(d0 > 0) and rel_l2(d2, d0) < 1e-5 and rel_l2(d1, d3) < 1e-5
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-10), (np.complex64, 2e-5)])
+def test_sample_operator_matches_oracle(ctx, dtype, tol):
+    """rs_apply_sample (fast-wave's apply_sample_factors + propagation per
+    grid row) against the oracle's Sample.apply, at the second x position and
+    with a cutoff."""
+    import ravesim
+    N, dx, wl = 2 ** 16, 1e-9, 4.1e-11
+    rng = np.random.default_rng(4)
+    grid = rng.integers(0, 3, size=(5, 24)).astype(np.int8)
+    dbl = [complex(5.1e-7, 1.6e-10), complex(3.6e-6, 2.1e-7)]
+    xs = [0.0, 1.7e-7]
+    fc = 0.35 / dx
+    u = rand_field(N, 9)
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.upload(u)
+    f.apply_sample(wl, ravesim.Sample(z_start=1e-3, pixel_size_x=4e-7, pixel_size_z=1.5e-6, grid=grid,
+                                      deltabeta=dbl, x_positions=xs), phase_step=1, cutoff_freq=fc)
+    got = f.download()
+    f.close()
+    p = bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=1e-6, detector_pixel_size_x=1e-7,
+                     detector_pixel_size_y=1.0, wl=wl, chunk_size=N)
+    ref = bw.Fields(u.astype(np.complex128))
+    s = bw.Sample(z_start=1e-3, pixel_size_x=4e-7, pixel_size_z=1.5e-6, grid=grid,
+                  db_list=np.array([0] + dbl, dtype=np.complex128), x_positions=np.array(xs))
+    bw.sample_apply(ref, s, p, fc, 1)
+    assert rel_l2(got, ref.u) < tol
