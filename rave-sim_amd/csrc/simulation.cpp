@@ -84,6 +84,9 @@ SimConfig parse_simdir(const std::string &sim_dir, int source_idx) {
       d.name = e[0][0].as_string("deltabeta_table material");
       d.density = e[0][1].as_double("deltabeta_table density");
       d.db = {e[1][0].as_double("delta"), e[1][1].as_double("beta")};
+      // a non-finite delta/beta would turn every sample it touches into NaN
+      if (!std::isfinite(d.db.real()) || !std::isfinite(d.db.imag()))
+        throw Error(2, "deltabeta of material " + d.name + " is not finite");
       table.push_back(d);
     }
   }

@@ -120,6 +120,7 @@ deltabeta_table:
     (lambda c, s: s[0]["deltabeta_table"].clear(), "not found"),
     (lambda c, s: c["elements"][0].__setitem__("type", "lens"), "not supported"),
     (lambda c, s: c["sim_params"].__setitem__("N", 1000), "power of two"),
+    (lambda c, s: s[0]["deltabeta_table"][0].__setitem__(1, [float("nan"), 0.0]), "not finite"),
 ])
 def test_config_errors_are_reported(tmp_path, mutation, needle):
     from ravesim import _lib
@@ -272,3 +273,51 @@ def test_fastwave_cli_arguments():
     r = subprocess.run([str(FASTWAVE), "x", "-s", "0", "--history_dz", "-1"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1 and "history_dz" in r.stderr
+
+
+def test_malformed_yaml_is_a_config_error(tmp_path):
+    """A config.yaml that is not YAML is reported, not crashed on."""
+    from ravesim import _lib
+    from ravesim.multisim import validate_simdir
+    sd = _write(tmp_path, cfg1(tmp_path, N=2 ** 12, slices=2, det_size=4e-6))
+    (sd / "config.yaml").write_text("sim_params: {N: 4096, dx: [1e-9\nelements: - : -\n\t]\n")
+    with pytest.raises(_lib.RsError) as e:
+        validate_simdir(sd, 0)
+    assert e.value.code == _lib.RS_ERR_CONFIG and str(e.value)
+
+
+def test_null_handles_are_rejected():
+    """Every entry point checks its handles and pointers and returns
+    RS_ERR_INVALID with a message, without touching a device."""
+    import ctypes as C
+    from ravesim import _lib
+    lib = _lib.lib
+    inv = _lib.RS_ERR_INVALID
+    checks = [
+        lambda: lib.rs_propagate(None, 2e-11, 1e-3, 1e30),
+        lambda: lib.rs_field_upload(None, None),
+        lambda: lib.rs_field_download(None, None),
+        lambda: lib.rs_field_copy(None, None),
+        lambda: lib.rs_point_source(None, 2e-11, 0.0, 1.0),
+        lambda: lib.rs_apply_grating(None, 2e-11, None, 0, 1e30),
+        lambda: lib.rs_apply_dba_grating(None, 2e-11, None, 0, 1e30),
+        lambda: lib.rs_apply_sample(None, 2e-11, None, 0, 1e30),
+        lambda: lib.rs_scale(None, 1.0, 0.0),
+        lambda: lib.rs_square_and_downsample(None, None, 1.0, None, 0),
+        lambda: lib.rs_run_simdir(None, b"x", 0, None),
+        lambda: lib.rs_batch_run(None, None, None, 0),
+        lambda: lib.rs_field_create(None, None, _lib.RS_C8, 0, None),
+        lambda: lib.rs_ctx_synchronize(None),
+        lambda: lib.rs_set_timing(None, 1),
+        lambda: lib.rs_autotune(None, 1024, _lib.RS_C8, 1, None, None),
+        lambda: lib.rs_set_split(None, 1024, _lib.RS_C8, 6, 4),
+        lambda: lib.rs_get_pass_stats(None, None, 0),
+        lambda: lib.rs_server_create(None, 0.0, 0, None),
+        lambda: lib.rs_server_submit(None, None, None),
+        lambda: lib.rs_server_poll(None, 0, None, None, 0, None, None),
+        lambda: lib.rs_server_wait(None, 0, 0.0, None),
+        lambda: lib.rs_server_stats(None, None),
+    ]
+    for k, call in enumerate(checks):
+        assert call() == inv, k
+        assert lib.rs_last_error().decode()
