@@ -292,10 +292,6 @@ __device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *
 // x-space operations
 // ---------------------------------------------------------------------------
 
-// Any x-space op on natural index n.  The grating coordinate and mask follow
-// big-wave bit for bit: x = linspace(x0, x0 + len*dx, len, endpoint=False)
-// per chunk, mask = mod(x/pitch + dc/2, 1) < dc, all in FP64 without
-// contraction (big-wave/optical_element.py:341-357).
 // Grating coordinate of sample n as big-wave forms it: np.linspace per chunk
 // (big-wave/optical_element.py:352-355), without contraction.
 __device__ __forceinline__ double grating_x(const XOp &op, int64_t n, int64_t N) {
@@ -329,6 +325,10 @@ __device__ __forceinline__ double grating_phase(double x, double pitch, double i
   return ph;
 }
 
+// Any x-space op on natural index n.  The grating coordinate and mask follow
+// big-wave bit for bit: x = linspace(x0, x0 + len*dx, len, endpoint=False)
+// per chunk, mask = mod(x/pitch + dc/2, 1) < dc, all in FP64 without
+// contraction (big-wave/optical_element.py:341-357).
 template <typename T2>
 __device__ __forceinline__ T2 apply_xop(const XOp &op, int64_t n, int64_t N, T2 v) {
 #pragma clang fp contract(off)
