@@ -3,6 +3,7 @@
 #include <cerrno>
 #include <cstdlib>
 #include <fstream>
+#include <map>
 #include <sstream>
 
 #include "engine.hpp"
@@ -69,6 +70,29 @@ size_t key_colon(const std::string &s) {
   return std::string::npos;
 }
 
+// anchors (&name) and aliases (*name), as PyYAML writes them for objects
+// that occur more than once (a material list shared by several elements)
+using Anchors = std::map<std::string, YNode>;
+
+bool name_end(char c) { return c == ' ' || c == '\t' || c == ',' || c == ']' || c == '}'; }
+
+// remove a leading "&name" from t and return the name ("" when t has none)
+std::string take_anchor(std::string &t) {
+  if (t.empty() || t[0] != '&') return "";
+  size_t e = 1;
+  while (e < t.size() && !name_end(t[e])) ++e;
+  const std::string name = t.substr(1, e - 1);
+  if (name.empty()) fail("empty anchor name");
+  t = trim(t.substr(e));
+  return name;
+}
+
+YNode alias_node(const Anchors &a, const std::string &name) {
+  const auto it = a.find(name);
+  if (it == a.end()) fail("alias *" + name + " before its anchor");
+  return it->second;
+}
+
 YNode scalar_node(const std::string &raw) {
   std::string s = trim(raw);
   // drop tags such as !!float or !!python/tuple
@@ -109,13 +133,28 @@ YNode scalar_node(const std::string &raw) {
 
 struct FlowParser {
   const std::string &s;
+  Anchors &anchors;
   size_t i = 0;
-  explicit FlowParser(const std::string &t) : s(t) {}
+  FlowParser(const std::string &t, Anchors &a) : s(t), anchors(a) {}
   void ws() {
     while (i < s.size() && (s[i] == ' ' || s[i] == '\t')) ++i;
   }
+  std::string name() {
+    const size_t b = ++i;
+    while (i < s.size() && !name_end(s[i])) ++i;
+    if (i == b) fail("empty anchor or alias name: " + s);
+    return s.substr(b, i - b);
+  }
   YNode value() {
     ws();
+    if (i < s.size() && s[i] == '*') return alias_node(anchors, name());
+    if (i < s.size() && s[i] == '&') {
+      const std::string a = name();
+      ws();
+      YNode n = (i < s.size() && s[i] != ',' && s[i] != ']' && s[i] != '}') ? value() : YNode{};
+      anchors[a] = n;
+      return n;
+    }
     if (i < s.size() && s[i] == '[') return seq();
     if (i < s.size() && s[i] == '{') return map();
     return scalar();
@@ -175,10 +214,10 @@ struct FlowParser {
   }
 };
 
-YNode inline_value(const std::string &v) {
+YNode inline_value(const std::string &v, Anchors &anchors) {
   const std::string t = trim(v);
-  if (!t.empty() && (t[0] == '[' || t[0] == '{')) {
-    FlowParser p(t);
+  if (!t.empty() && (t[0] == '[' || t[0] == '{' || t[0] == '&' || t[0] == '*')) {
+    FlowParser p(t, anchors);
     YNode n = p.value();
     p.ws();
     if (p.i != t.size()) fail("trailing characters after flow collection: " + t);
@@ -190,6 +229,7 @@ YNode inline_value(const std::string &v) {
 struct BlockParser {
   std::vector<Line> L;
   size_t pos = 0;
+  Anchors anchors;
 
   bool is_dash(const std::string &t) const { return t == "-" || (t.size() >= 2 && t[0] == '-' && t[1] == ' '); }
 
@@ -206,7 +246,7 @@ struct BlockParser {
     if (is_dash(t)) return sequence(ind);
     if (key_colon(t) != std::string::npos) return mapping(ind);
     ++pos;
-    return inline_value(take_flow(t));
+    return inline_value(take_flow(t), anchors);
   }
 
   YNode sequence(int ind) {
@@ -216,16 +256,20 @@ struct BlockParser {
       std::string rest = L[pos].text.substr(1);
       size_t k = 0;
       while (k < rest.size() && rest[k] == ' ') ++k;
-      if (k >= rest.size()) {
+      std::string item = rest.substr(k);
+      const std::string anchor = take_anchor(item);  // "- &name" before the item
+      k = rest.size() - item.size();
+      if (item.empty()) {
         ++pos;
         if (pos < L.size() && L[pos].indent > ind) n.seq.push_back(node(L[pos].indent));
         else n.seq.push_back(YNode{});
       } else {
         // re-read the remainder of this line as a node at its own column
         L[pos].indent = ind + 1 + (int)k;
-        L[pos].text = rest.substr(k);
+        L[pos].text = item;
         n.seq.push_back(node(L[pos].indent));
       }
+      if (!anchor.empty()) anchors[anchor] = n.seq.back();
     }
     return n;
   }
@@ -238,10 +282,18 @@ struct BlockParser {
       const size_t c = key_colon(t);
       if (c == std::string::npos) fail("expected 'key:' at line " + std::to_string(L[pos].no));
       const std::string key = scalar_node(t.substr(0, c)).s;
-      const std::string val = c + 1 < t.size() ? trim(t.substr(c + 1)) : std::string();
+      std::string val = c + 1 < t.size() ? trim(t.substr(c + 1)) : std::string();
       ++pos;
+      // "key: &name" followed by a block node on the next lines
+      std::string anchor;
+      if (!val.empty() && val[0] == '&') {
+        std::string rest = val;
+        anchor = take_anchor(rest);
+        if (rest.empty()) val.clear();
+        else anchor.clear();  // an inline anchored value: the flow parser records it
+      }
       if (!val.empty()) {
-        n.map.emplace_back(key, inline_value(take_flow(val)));
+        n.map.emplace_back(key, inline_value(take_flow(val), anchors));
       } else if (pos < L.size() && L[pos].indent > ind) {
         n.map.emplace_back(key, node(L[pos].indent));
       } else if (pos < L.size() && L[pos].indent == ind && is_dash(L[pos].text)) {
@@ -249,6 +301,7 @@ struct BlockParser {
       } else {
         n.map.emplace_back(key, YNode{});
       }
+      if (!anchor.empty()) anchors[anchor] = n.map.back().second;
     }
     return n;
   }

@@ -137,3 +137,59 @@ def dba_case(N=2 ** 16, dtype="c16", steps=(0.0, 1.05e-6), chunk=None):
     subs = [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": 0.0},
              "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
     return cfg, subs, [math.pi / 2] * 3
+
+
+def random_case(tmp: Path, seed: int):
+    """A seeded random valid sim dir for cross-checks: N = 2^12..2^15, a plane
+    wave or a point source, 1-3 elements drawn from grating / DBA grating /
+    sample (one of them phase stepped over 2-3 positions), random substrates,
+    chunk sizes, cutoff angles and detector.  Returns (config, sub-configs,
+    cutoff angles, sample grid or None)."""
+    rng = np.random.default_rng(seed)
+    N = int(2 ** rng.integers(12, 16))
+    dx = float(rng.choice([1e-9, 2e-9]))
+    energy = float(rng.uniform(20e3, 60e3))
+    chunk = int(rng.choice([N, N // 4, 1000, 3000]))
+    n_el = int(rng.integers(1, 4))
+    stepped = int(rng.integers(0, n_el))
+    n_steps = int(rng.integers(2, 4))
+    mats = [["Si", 2.34], ["Au", 19.32]]
+    z = float(rng.uniform(0.002, 0.01))
+    els, grid = [], None
+    for i in range(n_el):
+        kind = str(rng.choice(["grating", "dba", "sample"])) if grid is None else str(rng.choice(["grating", "dba"]))
+        pitch = float(rng.uniform(0.3e-6, 1.5e-6))
+        xs = [float(rng.uniform(-pitch, pitch)) for _ in range(n_steps if i == stepped else 1)]
+        sub = float(rng.choice([0.0, float(rng.uniform(1e-6, 5e-6))]))
+        if kind == "grating":
+            e = grating(pitch, z, float(rng.uniform(2e-6, 10e-6)), int(rng.integers(1, 5)), mats[int(rng.integers(0, 2))],
+                        x_positions=xs, dc=(float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7))),
+                        substrate=sub, mat_b=None if rng.random() < 0.5 else mats[0],
+                        mat_s=mats[0] if sub else None)
+        elif kind == "dba":
+            e = {"type": "dba_grating", "pitch0": pitch, "pitch1": pitch * float(rng.choice([2.0, 3.0, 2.5])),
+                 "dc0": [float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7))],
+                 "dc1": [float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7))], "z_start": z,
+                 "thickness": float(rng.uniform(2e-6, 8e-6)), "nr_steps": int(rng.integers(1, 5)), "x_positions": xs,
+                 "substrate_thickness": sub, "mat_a": mats[1], "mat_b": None, "mat_substrate": mats[0] if sub else None}
+        else:
+            rows, cols = int(rng.integers(1, 4)), int(rng.integers(3, 12))
+            grid = rng.integers(0, 3, size=(rows, cols)).astype(np.int8)
+            e = {"type": "sample", "z_start": z, "pixel_size_x": float(rng.uniform(1e-7, 6e-7)),
+                 "pixel_size_z": float(rng.uniform(1e-6, 3e-6)), "grid_path": "grid.npy", "materials": mats,
+                 "x_positions": xs}
+        els.append(e)
+        thick = (e["pixel_size_z"] * grid.shape[0]) if kind == "sample" else e["thickness"] + e["substrate_thickness"]
+        z = z + thick + float(rng.uniform(0.0, 0.01))
+    z_det = z + float(rng.uniform(0.0, 0.02))
+    det = min(N * dx / 2, float(rng.uniform(2e-6, 2e-5)))
+    cfg = {"sim_params": sim_params(N, dx, z_det, det, float(rng.choice([2e-7, 5e-7, 1e-6])), 1.0, chunk),
+           "use_disk_vector": False, "save_final_u_vectors": False, "dtype": "c16", "elements": els}
+    if rng.random() < 0.5:
+        src = {"type": "point", "x": float(rng.uniform(-1e-7, 1e-7)), "z": -float(rng.uniform(0.5, 2.0))}
+    else:
+        src = {"type": "vector", "input_path": write_ones(tmp / "ones.npy", N), "z": 0.0}
+    subs = [{"energy": energy, "source": src,
+             "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
+    ang = sorted(float(rng.choice([float(rng.uniform(0.2e-3, 2e-3)), math.pi / 2])) for _ in range(n_el + 1))
+    return cfg, subs, ang, grid

@@ -321,3 +321,60 @@ def test_null_handles_are_rejected():
     for k, call in enumerate(checks):
         assert call() == inv, k
         assert lib.rs_last_error().decode()
+
+
+def test_yaml_anchors_and_aliases(tmp_path):
+    """Objects shared in a config dict are written by PyYAML as anchors and
+    aliases (a material used by several gratings, a repeated element); they
+    parse to the same configuration as the written-out form."""
+    from ravesim.multisim import validate_simdir
+    from ravesim.simdir import write_simdir
+    from simcases import grating
+    si = ["Si", 2.34]
+    g = grating(1e-6, 0.01, 4e-6, 3, si, substrate=1e-6, mat_s=si)
+    def cfg(els):
+        return {"sim_params": {"N": 4096, "dx": 1e-9, "z_detector": 0.05, "detector_size": 1e-6,
+                               "detector_pixel_size_x": 1e-7, "detector_pixel_size_y": 1.0, "chunk_size": 4096},
+                "use_disk_vector": False, "save_final_u_vectors": False, "dtype": "c16", "elements": els}
+    subs = [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": -1.0},
+             "deltabeta_table": [[si, [5.1e-7, 1.6e-10]]]}]
+    g2 = dict(g, z_start=0.02)
+    shared = write_simdir(tmp_path / "a", cfg([g, g2]), subs, [0.5] * 3)
+    text = (shared / "config.yaml").read_text()
+    assert "&id" in text and "*id" in text
+    import copy
+    plain = write_simdir(tmp_path / "b", copy.deepcopy(cfg([g, g2])) | {"elements": [
+        {k: (list(v) if isinstance(v, list) else v) for k, v in e.items()} for e in (g, g2)]},
+        [{"energy": 30000.0, "source": {"type": "point", "x": 0.0, "z": -1.0},
+          "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]]]}], [0.5] * 3)
+    assert "&id" not in (plain / "config.yaml").read_text()
+    assert list(validate_simdir(shared, 0)) == list(validate_simdir(plain, 0))
+    # written by hand: an anchored block mapping in a sequence, a flow
+    # mapping with an anchor, aliases of a list inside later elements, and an
+    # alias of a block mapping
+    top = text[:text.index("elements:")]
+    (shared / "config.yaml").write_text(top + """elements:
+- &first
+  type: grating
+  pitch: 1.0e-06
+  dc: [0.5, 0.5]
+  z_start: 0.001
+  thickness: 1.0e-06
+  nr_steps: 2
+  x_positions: [0.0]
+  substrate_thickness: 0.0
+  mat_a: &si [Si, 2.34]
+- &second {type: grating, pitch: 2.0e-06, dc: [0.4, 0.4], z_start: 0.01, thickness: 2.0e-06, nr_steps: 1, x_positions: [0.0], substrate_thickness: 1.0e-06, mat_a: *si, mat_substrate: *si}
+- type: grating
+  pitch: 1.5e-06
+  dc: [0.5, 0.5]
+  z_start: 0.02
+  thickness: 1.0e-06
+  nr_steps: 1
+  x_positions: [0.0]
+  substrate_thickness: 0.0
+  mat_a: *si
+unused_copy: *first
+""")
+    (shared / "computed.yaml").write_text("cutoff_angles: [0.5, 0.5, 0.5, 0.5]\n")
+    assert list(validate_simdir(shared, 0))[4] == 3
