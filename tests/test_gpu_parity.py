@@ -607,3 +607,65 @@ def test_random_configs_match_oracle(ctx, tmp_path, seed, dtype, tol):
     hist = np.load(sd / "00000000" / "history.npy")
     assert np.array_equal(np.load(sd / "00000000" / "history_z.npy"), np.asarray(h.zs, dtype=np.float64))
     assert rel_l2(hist, h.get_history()) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-10), (np.complex64, 1e-4)])
+@pytest.mark.parametrize("seed", range(8))
+def test_random_batches_match_oracle(ctx, seed, dtype, tol):
+    """Seeded random lockstep batches: one structure per batch (source kind,
+    gratings, slice counts, substrates), per-member energies, pitches, duty
+    cycles, positions, thicknesses, materials and cutoffs, random weights and
+    output rows, random lanes and group sizes; against the oracle's weighted
+    per-member sums."""
+    from ravesim.core import BatchMember, run_batch
+    import ravesim
+    rng = np.random.default_rng(100 + seed)
+    N = int(2 ** rng.integers(12, 16))
+    dx = 1e-9
+    chunk = int(rng.choice([N, 1000]))
+    ng = int(rng.integers(1, 3))
+    steps = [int(rng.integers(1, 4)) for _ in range(ng)]
+    has_sub = [bool(rng.random() < 0.5) for _ in range(ng)]
+    z0 = [0.01 + 0.01 * k for k in range(ng)]
+    z_det = z0[-1] + 0.01 + float(rng.uniform(0, 0.02))
+    point = bool(rng.random() < 0.5)
+    n_out = int(rng.integers(1, 4))
+    det, px = min(N * dx / 2, 1e-5), 2e-7
+    ms, ref = [], np.zeros((n_out, int(det // px)))
+    for _ in range(int(rng.integers(2, 9))):
+        e = float(rng.uniform(20e3, 60e3))
+        wl = ravesim.energy_to_wavelength(e)
+        gs, ogs = [], []
+        for k in range(ng):
+            pitch = float(rng.uniform(0.3e-6, 1.5e-6))
+            dc = (float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.3, 0.7)))
+            xs = [float(rng.uniform(-pitch, pitch))]
+            th = float(rng.uniform(1e-6, 5e-6))
+            sub = float(rng.uniform(1e-6, 3e-6)) if has_sub[k] else 0.0
+            dba, dbb, dbs = (complex(rng.uniform(1e-7, 4e-6), rng.uniform(0, 2e-7)) for _ in range(3))
+            gs.append(ravesim.Grating(pitch=pitch, dc=dc, z_start=z0[k], thickness=th, nr_steps=steps[k],
+                                      x_positions=xs, substrate_thickness=sub, db_a=dba, db_b=dbb,
+                                      db_substrate=dbs))
+            ogs.append(bw.Grating(pitch=pitch, dc=dc, z_start=z0[k], thickness=th, nr_steps=steps[k],
+                                  x_positions=np.array(xs), substrate_thickness=sub, db_a=dba, db_b=dbb,
+                                  db_substrate=dbs))
+        ang = sorted(float(rng.choice([float(rng.uniform(0.2e-3, 2e-3)), math.pi / 2])) for _ in range(ng + 1))
+        w, r = float(rng.uniform(0.1, 1.0)), int(rng.integers(0, n_out))
+        xsrc = float(rng.uniform(-1e-7, 1e-7))
+        ms.append(BatchMember(wl=wl, gratings=gs, cutoff_angles=ang, weight=w, out_index=r,
+                              plane_wave=not point, source_x=xsrc if point else 0.0,
+                              source_z=-0.5 if point else z0[0]))
+        p = bw.SimParams(N=N, dx=dx, z_detector=z_det, detector_size=det, detector_pixel_size_x=px,
+                         detector_pixel_size_y=1.0, wl=wl, chunk_size=chunk)
+        src = (bw.PointSource(x=xsrc, z=-0.5) if point
+               else bw.VectorSource(vec=np.ones(N, dtype=np.complex128), z=z0[0]))
+        ref[r] += w * bw.run_simulation(p, src, ogs, ang)[0]
+    got = run_batch(ctx, N, dx, ms, n_outputs=n_out, dtype=dtype, chunk_size=chunk, det_size=det, det_px=px,
+                    det_py=1.0, z_detector=z_det, lanes=int(rng.integers(0, 4)),
+                    max_resident=int(rng.integers(0, 4)))
+    assert got.shape == ref.shape
+    for k in range(n_out):
+        if np.any(ref[k]):
+            assert rel_l2(got[k], ref[k]) < tol, f"row {k}"
+        else:
+            assert not np.any(got[k])
