@@ -669,3 +669,54 @@ def test_random_batches_match_oracle(ctx, seed, dtype, tol):
             assert rel_l2(got[k], ref[k]) < tol, f"row {k}"
         else:
             assert not np.any(got[k])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_splits_match_oracle(seed):
+    """Random N and random valid digit-group splits (two or three groups,
+    wide columns included): a grating (slices, substrate, multi-chunk x) and a
+    propagation with cutoff through the operators, against the oracle."""
+    import ravesim
+    from ravesim import _lib
+    rng = np.random.default_rng(200 + seed)
+    dtype = np.complex128 if seed % 2 else np.complex64
+    f64 = dtype == np.complex128
+    lgN = int(rng.integers(10, 21))
+    N, dx, wl = 2 ** lgN, 2e-9, 2.7e-11
+    c = ravesim.Context(0)
+    split = None
+    for _ in range(50):
+        r = int(rng.integers(6, (12 if f64 else 13) + 1))
+        a = int(rng.integers(4, (12 if f64 else 13) + 1))
+        try:
+            c.set_split(N, dtype, r, a)
+        except _lib.RsError:
+            continue
+        split = (r, a)
+        break
+    assert split is not None
+    chunk = int(rng.choice([N, max(N // 3, 64)]))
+    u = rand_field(N, seed)
+    pitch = float(rng.uniform(0.5e-6, 3e-6))
+    g = ravesim.Grating(pitch=pitch, dc=(0.35, 0.6), z_start=0.0, thickness=6e-6, nr_steps=3,
+                        x_positions=[1.1e-7], substrate_thickness=2e-6, db_a=complex(3e-6, 1e-7),
+                        db_substrate=complex(5e-7, 1e-9))
+    fc = float(rng.uniform(0.2, 0.5)) / dx
+    f = ravesim.Field(c, N, dx, dtype=dtype, chunk_size=chunk)
+    f.upload(u)
+    f.apply_grating(wl, g, cutoff_freq=fc)
+    f.propagate(wl, 1.3e-3, fc)
+    got = f.download()
+    f.close()
+    c.close()
+    p = bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=1e-6, detector_pixel_size_x=1e-7,
+                     detector_pixel_size_y=1.0, wl=wl, chunk_size=chunk)
+    ref = bw.Fields(u.astype(np.complex128))
+    og = bw.Grating(pitch=pitch, dc=(0.35, 0.6), z_start=0.0, thickness=6e-6, nr_steps=3,
+                    x_positions=np.array([1.1e-7]), substrate_thickness=2e-6, db_a=complex(3e-6, 1e-7),
+                    db_substrate=complex(5e-7, 1e-9))
+    bw.grating_apply(ref, og, p, fc, 0)
+    bw.propagate(ref, dx, wl, 1.3e-3, chunk, fc)
+    err = rel_l2(got, ref.u)
+    print(f"2^{lgN} {'c16' if f64 else 'c8'} split {split} chunk {chunk}: {err:.2e}")
+    assert err < (1e-10 if f64 else 2e-5)
