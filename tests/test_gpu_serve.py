@@ -91,3 +91,26 @@ def test_admission_with_small_budget_completes():
     assert s.stats()["max_resident_jobs"] == 1
     s.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_request_mix_matches_oracle(server, seed):
+    """A seeded mix of requests submitted together: N = 2^12..2^15, 1-3
+    energies with random weights, 2-4 phase steps, 2-4 slices per grating,
+    complex64 or complex128; every result against its oracle sweep."""
+    from ravesim.serve import phase_sweep
+    rng = np.random.default_rng(300 + seed)
+    reqs = []
+    for _ in range(int(rng.integers(3, 7))):
+        dt = np.complex128 if rng.random() < 0.5 else np.complex64
+        ne = int(rng.integers(1, 4))
+        reqs.append(phase_sweep(int(2 ** rng.integers(12, 16)), list(rng.uniform(20e3, 60e3, ne)),
+                                n_steps=int(rng.integers(2, 5)), slices=int(rng.integers(2, 5)),
+                                det_size=2e-6, dtype=dt, weights=list(rng.uniform(0.2, 1.0, ne))))
+    ids = [server.submit(r) for r in reqs]
+    for r, i in zip(reqs, ids):
+        got = server.result(i).rows
+        ref = oracle_sweep(r)
+        tol = 1e-10 if r.dtype == np.complex128 else 1e-4
+        assert got.shape == ref.shape
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol, (r.N, r.dtype)
