@@ -137,14 +137,24 @@ def test_point_source_matches_oracle(ctx, dtype):
     assert rel_l2(f.download(), ref) < tol
 
 
-@pytest.mark.parametrize("N,size,px", [(8, 3.0, 1.0), (2 ** 16, 1e-4, 1e-6), (2 ** 16, 5e-5, 3e-7),
-                                       (2 ** 12, 4e-6, 1e-7)])
+def test_detector_known_answer(ctx):
+    """big-wave/test.py:209-225 on an 8-sample field; fields here need
+    N >= 64, so the 8 samples sit at the same x positions (n = -4..3) of a
+    64-sample field whose other samples are zero."""
+    import ravesim
+    u = np.zeros(64, dtype=np.complex128)
+    u[28:36] = np.arange(8)
+    f = ravesim.Field(ctx, 64, 0.5, dtype=np.complex128)
+    f.upload(u)
+    got = f.square_and_downsample(3.0, 1.0, 1.0, 1.0)
+    assert np.allclose(got, [1.25, 12.5, 15.249999999999998], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("N,size,px", [(2 ** 16, 1e-4, 1e-6), (2 ** 16, 5e-5, 3e-7), (2 ** 12, 4e-6, 1e-7)])
 def test_detector_matches_histogram(ctx, N, size, px):
     import ravesim
-    dx = 0.5 if N == 8 else 2e-9
-    u = np.arange(8, dtype=np.complex128) if N == 8 else rand_field(N, 11)
-    if N < 64:
-        pytest.skip("fields need N >= 64")
+    dx = 2e-9
+    u = rand_field(N, 11)
     f = ravesim.Field(ctx, N, dx, dtype=np.complex128)
     f.upload(u)
     got = f.square_and_downsample(size, px, 1.0, 1.0)
