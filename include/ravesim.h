@@ -256,6 +256,10 @@ typedef struct rs_batch_desc {
   int32_t use_graph;       /* capture each group's launch sequence in a hipGraph */
   int32_t lanes;           /* streams that groups alternate over (1..4); 0 = 2 with the default
                               grouping, 1 otherwise */
+  int32_t spans;           /* three-group plans: run each propagation's middle, row and middle passes
+                              over this many consecutive sub-ranges of the fields (a power of two, at
+                              most the top group's length); 0 = the fewest whose share of a group's
+                              fields fits the 256 MiB last-level cache */
 } rs_batch_desc;
 
 /* Run every member; out[nr_outputs * n_px] receives sum over members of

@@ -671,6 +671,8 @@ int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out
     rc.use_graph = d->use_graph != 0;
     if (d->lanes < 0 || d->lanes > 4) throw rs::Error(RS_ERR_INVALID, "lanes must be 0..4");
     rc.lanes = d->lanes;
+    if (d->spans < 0) throw rs::Error(RS_ERR_INVALID, "spans must be >= 0");
+    rc.spans = d->spans;
     std::vector<double> det = rs::run_members(*ctx->ctx, rc, ms);
     std::memcpy(out, det.data(), det.size() * sizeof(double));
   });

@@ -512,6 +512,21 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   // the member's host arrays need not outlive the copies
   if (!sample_bufs.empty()) hip_check(hipStreamSynchronize(ctx.stream), "sample upload sync");
 
+  // spans of the middle-row-middle sequence (three-group plans): a group
+  // whose fields overflow the last-level cache runs it over halves, quarters,
+  // ... of the fields that fit
+  int spans = 1;
+  if (rc.mode == 0 && plan.ngroups == 3 && plan.geom_mid().xcd_segs == 1) {
+    if (rc.spans > 0) {
+      if (rc.spans & (rc.spans - 1) || rc.spans > (1 << plan.lgA))
+        throw Error(1, "spans must be a power of two no larger than the top group");
+      spans = rc.spans;
+    } else {
+      const size_t gbytes = (size_t)lanes * width * fbytes;
+      while (spans < (1 << plan.lgA) && gbytes / (size_t)spans > kMallBytes) spans *= 2;
+    }
+  }
+
   std::vector<std::unique_ptr<Exec<S>>> groups;
   size_t nfa = 0, nda = 0;
   for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {
@@ -519,6 +534,7 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     const int lane = (int)(gi % lanes);
     auto ex = std::make_unique<Exec<S>>(ctx, rc, plan);
     ex->st = lane_st[lane];
+    ex->spans = spans;
     for (size_t i = g0; i < g1; ++i) {
       ex->mem.push_back(&members[i]);
       ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);

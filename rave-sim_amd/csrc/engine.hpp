@@ -141,6 +141,7 @@ struct RunConfig {
   int max_resident = 0;      // members per group; 0 = as many as fit the last-level cache together
   bool use_graph = false;
   int lanes = 0;             // streams groups alternate over; 0 = 2 with the default grouping
+  int spans = 0;             // sub-ranges of the middle-row-middle passes (3 groups); 0 = by cache size
   int lgR = -1;              // row length override (autotuner), -1 = tuned or default
   int lgA = -1;              // top column length override
   void *const *fields = nullptr;  // caller-owned field buffers (one per member), else allocated

@@ -162,6 +162,10 @@ template <typename S> struct Exec {
   const int64_t *bounds = nullptr;
   int64_t npx = 0;
   hipStream_t st;  // stream the launches go to (the context stream by default)
+  // three-group plans: the middle, row and middle passes of a propagation run
+  // as `spans` consecutive sub-ranges of the fields, each small enough to stay
+  // in the last-level cache from its first pass to its last
+  int spans = 1;
 
   Exec(Context &c, const RunConfig &r, Plan &p) : ctx(c), rc(r), plan(p), st(c.stream) {}
 
@@ -215,24 +219,33 @@ template <typename S> struct Exec {
       }
       fiber(plan.lgA, false, plan.geom_top(), a);
     }
-    if (plan.ngroups == 3) {
-      auto a = blank();
-      for (int i = 0; i < K; ++i) a[i].do_fwd = 1;
-      fiber(plan.lgB, false, plan.geom_mid(), a);
-    }
-    {
-      auto a = blank();
-      for (int i = 0; i < K; ++i) {
-        a[i].do_fwd = 1;
-        a[i].do_inv = 1;
-        a[i].k = make_kop(pa[i], rc.N, true);
+    const int ns = plan.ngroups == 3 ? spans : 1;
+    const int64_t len = rc.N / ns;
+    for (int c = 0; c < ns; ++c) {
+      Geom gm = plan.geom_mid(), gr = plan.geom_row();
+      if (ns > 1) {
+        gm.span_off = gr.span_off = c * len;
+        gm.span_len = gr.span_len = len;
       }
-      fiber(plan.lgR, true, plan.geom_row(), a);
-    }
-    if (plan.ngroups == 3) {
-      auto a = blank();
-      for (int i = 0; i < K; ++i) a[i].do_inv = 1;
-      fiber(plan.lgB, false, plan.geom_mid(), a);
+      if (plan.ngroups == 3) {
+        auto a = blank();
+        for (int i = 0; i < K; ++i) a[i].do_fwd = 1;
+        fiber(plan.lgB, false, gm, a);
+      }
+      {
+        auto a = blank();
+        for (int i = 0; i < K; ++i) {
+          a[i].do_fwd = 1;
+          a[i].do_inv = 1;
+          a[i].k = make_kop(pa[i], rc.N, true);
+        }
+        fiber(plan.lgR, true, gr, a);
+      }
+      if (plan.ngroups == 3) {
+        auto a = blank();
+        for (int i = 0; i < K; ++i) a[i].do_inv = 1;
+        fiber(plan.lgB, false, gm, a);
+      }
     }
   }
 
@@ -490,8 +503,10 @@ template <typename S> struct Exec {
     auto t_begin = [&](int cls, double bytes) {
       if (!timed) return;
       hipEvent_t a, b;
-      hip_check(hipEventCreate(&a), "event");
-      hip_check(hipEventCreate(&b), "event");
+      // timing only: no system-scope fence (cache write-back) at each record;
+      // the times are read after the stream has been synchronised
+      hip_check(hipEventCreateWithFlags(&a, hipEventDisableSystemFence), "event");
+      hip_check(hipEventCreateWithFlags(&b, hipEventDisableSystemFence), "event");
       ev.push_back(a);
       ev.push_back(b);
       ev_meta.emplace_back(cls, bytes);
@@ -506,7 +521,8 @@ template <typename S> struct Exec {
         case LKind::Fiber: {
           begin_capture();
           const int cls = l.row ? PC_ROW : (l.geom.top ? PC_TOP : PC_MID);
-          t_begin(cls, 2.0 * (double)rc.N * (double)esz * l.nf);
+          const double ns = (double)(l.geom.span_len ? l.geom.span_len : rc.N);
+          t_begin(cls, 2.0 * ns * (double)esz * l.nf);
           hip_check(launch_fiber_pass<S>(l.lgL, l.row, l.xv, fa + l.arg_off, l.nf, l.geom, st), "fiber pass");
           t_end();
           break;

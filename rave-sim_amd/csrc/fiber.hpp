@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       __syncthreads();
     }
   }
-  const int64_t tile = blockIdx.x;
+  const int64_t tile = g.tile0 + blockIdx.x;
   const TileMap<L, C, ROW> tm(g, tile, t);
   const int c = tm.c, q = tm.q;
   const int64_t mst = tm.mst;
@@ -252,15 +252,34 @@ hipError_t launch_mask_fill_t(int lgL, const XOp &x, const Geom &g, void *bits, 
   }
 }
 
+// tiles of `per` samples each covering the launch's span: first tile id into
+// gs.tile0, count into nblk; false when the span does not split into whole
+// tiles (or into whole XCD groups of tiles, when the pass regroups them)
+static inline bool span_tiles(const Geom &g, int64_t per, Geom &gs, int64_t &nblk) {
+  gs = g;
+  const int64_t off = g.span_len ? g.span_off : 0;
+  const int64_t len = g.span_len ? g.span_len : g.N;
+  if (off < 0 || len <= 0 || off + len > g.N || off % per || len % per) return false;
+  gs.tile0 = off / per;
+  nblk = len / per;
+  if (g.span_len && g.xcd_segs > 1) {
+    const int64_t G = 8 * (int64_t)g.xcd_segs;
+    if (gs.tile0 % G || nblk % G) return false;
+  }
+  return true;
+}
+
 // columns per workgroup for a column pass of fiber length L: 16 columns of
 // complex64 (128 B per row segment) / 8 of complex128, more for short fibers
 template <typename S, int L, int XV>
 static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
   constexpr int C = col_fibers(L, sizeof(S) == 8);
   if (g.S % C != 0) return hipErrorInvalidValue;
-  const int64_t nblk = g.N / ((int64_t)L * C);
+  Geom gs;
+  int64_t nblk;
+  if (!span_tiles(g, (int64_t)L * C, gs, nblk)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk, (unsigned)nf);
-  hipLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, args, g);
+  hipLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, args, gs);
   return hipGetLastError();
 }
 
@@ -272,12 +291,15 @@ static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipSt
   constexpr int SPAN = sizeof(S) == 8 ? 2048 : 4096;
   constexpr int C = L >= SPAN ? 1 : SPAN / L;
   const int64_t rows = g.N / L;
-  if (rows % C == 0) {
-    dim3 grid((unsigned)(rows / C), (unsigned)nf);
-    hipLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, args, g);
+  Geom gs;
+  int64_t nblk;
+  if (rows % C == 0 && span_tiles(g, (int64_t)L * C, gs, nblk)) {
+    dim3 grid((unsigned)nblk, (unsigned)nf);
+    hipLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, args, gs);
   } else {
-    dim3 grid((unsigned)rows, (unsigned)nf);
-    hipLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, args, g);
+    if (!span_tiles(g, (int64_t)L, gs, nblk)) return hipErrorInvalidValue;
+    dim3 grid((unsigned)nblk, (unsigned)nf);
+    hipLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, args, gs);
   }
   return hipGetLastError();
 }

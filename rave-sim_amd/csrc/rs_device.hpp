@@ -106,6 +106,11 @@ struct Geom {
   const void *tw_lo;
   const void *tw_fib;  // W_L^j, j < L, then the small-stage tables (rs_kernels.hpp)
   const void *cis_tab;  // complex64 plans: exp(2 pi i j / 256), j < 256 (row pass phase)
+  // samples [span_off, span_off + span_len) of every field (span_len 0: the
+  // whole field); a span is a whole number of top-digit blocks, so that the
+  // pass's tiles inside it are the contiguous ids tile0 .. tile0 + grid - 1
+  // (tile0 is set by the launcher)
+  int64_t span_off, span_len, tile0;
 };
 
 }  // namespace rs

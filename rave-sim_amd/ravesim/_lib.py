@@ -97,7 +97,7 @@ class rs_batch_desc(C.Structure):
                 ("dtype", C.c_int32), ("mode", C.c_int32), ("nr_gratings", C.c_int32),
                 ("nr_members", C.c_int32), ("members", C.POINTER(rs_batch_member)),
                 ("nr_outputs", C.c_int32), ("max_resident", C.c_int32),
-                ("use_graph", C.c_int32), ("lanes", C.c_int32)]
+                ("use_graph", C.c_int32), ("lanes", C.c_int32), ("spans", C.c_int32)]
 
 
 class rs_request(C.Structure):

@@ -287,9 +287,10 @@ def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *
               det_size: float, det_px: float, det_py: float, z_detector: float,
               n_outputs: int = 1, dtype=np.complex64, chunk_size: Optional[int] = None,
               plain: bool = False, max_resident: int = 0, use_graph: bool = False,
-              lanes: int = 0) -> np.ndarray:
+              lanes: int = 0, spans: int = 0) -> np.ndarray:
     """Run every member in lockstep batches; returns (n_outputs, n_px) float64
-    with sum over members of weight * detector row."""
+    with sum over members of weight * detector row.  `spans` (three-group
+    plans): sub-ranges the middle-row-middle passes run over, 0 = by cache size."""
     if not members:
         raise ValueError("no members")
     ng = len(members[0].gratings)
@@ -316,7 +317,7 @@ def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *
         dtype=_lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16,
         mode=_lib.RS_MODE_PLAIN if plain else _lib.RS_MODE_FUSED,
         nr_gratings=ng, nr_members=len(members), members=cm, nr_outputs=n_outputs,
-        max_resident=max_resident, use_graph=1 if use_graph else 0, lanes=int(lanes))
+        max_resident=max_resident, use_graph=1 if use_graph else 0, lanes=int(lanes), spans=int(spans))
     det = _lib.rs_detector(size=det_size, pixel_x=det_px, pixel_y=1.0)
     npx = lib.rs_detector_pixels(C.byref(det))
     out = np.zeros((n_outputs, npx), dtype=np.float64)

@@ -249,6 +249,45 @@ def test_forced_splits_match_oracle(N, dtype, lgR, lgA):
     c.close()
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_spans_give_identical_rows(dtype):
+    """The middle, row and middle passes of each propagation run over 1..64
+    consecutive sub-ranges of the fields (a 2^26 complex64 field runs over two
+    by default): every sample goes through the same tile either way, so the
+    detector rows are identical, and they match the oracle."""
+    import ravesim
+    from ravesim import _lib
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 18, 2e-9
+    c = ravesim.Context(0)
+    c.set_split(N, dtype, 8, 6)  # row 256 | top 64 | middle 16
+    wl = ravesim.energy_to_wavelength(30000.0)
+    gcfg = [dict(pitch=1.3e-6, dc=(0.45, 0.45), z_start=0.0, thickness=8e-6, nr_steps=3, db_a=complex(2e-6, 3e-8)),
+            dict(pitch=1.3e-6, dc=(0.5, 0.5), z_start=0.05, thickness=8e-6, nr_steps=2, db_a=complex(2e-6, 3e-8))]
+    gr = [ravesim.Grating(**g) for g in gcfg]
+    # lockstep members share one operation structure: two point sources
+    ms = [BatchMember(wl=wl, gratings=gr, cutoff_angles=[0.2] * 3, source_x=-1e-7, source_z=-0.2, weight=0.7),
+          BatchMember(wl=wl * 1.1, gratings=gr, cutoff_angles=[0.2] * 3, source_x=2e-7, source_z=-0.1)]
+    kw = dict(det_size=2e-4, det_px=1e-6, det_py=1.0, z_detector=0.1 + 8e-6, dtype=dtype, max_resident=2)
+    base = run_batch(c, N, dx, ms, spans=1, **kw)
+    for sp in (2, 8, 64):
+        assert np.array_equal(run_batch(c, N, dx, ms, spans=sp, **kw), base), sp
+    for bad in (3, 128):
+        with pytest.raises(_lib.RsError, match="spans"):
+            run_batch(c, N, dx, ms, spans=bad, **kw)
+    c.close()
+    # the same setup through the oracle, member by member (complex128)
+    ogs = [bw.Grating(x_positions=np.array([0.0]), substrate_thickness=0.0, **g) for g in gcfg]
+    ref = 0
+    for m in ms:
+        p = bw.SimParams(N=N, dx=dx, z_detector=kw["z_detector"], detector_size=kw["det_size"],
+                         detector_pixel_size_x=kw["det_px"], detector_pixel_size_y=1.0, wl=m.wl, chunk_size=N)
+        src = bw.PointSource(x=m.source_x, z=m.source_z)
+        ref = ref + m.weight * bw.run_simulation(p, src, ogs, m.cutoff_angles)[0]
+    tol = 1e-9 if dtype == np.complex128 else 1e-4
+    assert rel_l2(base[0], ref) < tol
+
+
 def test_autotune_per_batch_size():
     """Splits tuned for one field and for a lockstep batch are both valid
     plans; a batch run under the batch-tuned split matches the default

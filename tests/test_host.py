@@ -26,6 +26,35 @@ def test_library_exports_every_declared_symbol():
     assert _lib.lib.rs_version() >= 100
 
 
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror in ravesim._lib has the size and the field offsets
+    of its declaration in include/ravesim.h, compiled by the host C compiler
+    (a field missing from either side fails to compile or to match)."""
+    import ctypes
+    import subprocess
+    from ravesim import _lib
+    header = (ROOT / "include" / "ravesim.h").read_text()
+    structs = re.findall(r"^typedef struct (rs_\w+) \{", header, re.M)
+    assert len(structs) >= 10
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "ravesim.h"', "int main(void) {"]
+    for s in structs:
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for name, _ in getattr(_lib, s)._fields_:
+            lines.append(f'printf("{s}.{name} %zu\\n", offsetof({s}, {name}));')
+    lines.append("return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-I", str(ROOT / "include"), str(tmp_path / "layout.c"),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.splitlines())
+    for s in structs:
+        cls = getattr(_lib, s)
+        assert int(got[s]) == ctypes.sizeof(cls), s
+        for name, _ in cls._fields_:
+            assert int(got[f"{s}.{name}"]) == getattr(cls, name).offset, f"{s}.{name}"
+
+
 def test_failure_without_device_is_an_error_code():
     import ravesim
     from ravesim import _lib
