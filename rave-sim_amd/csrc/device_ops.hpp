@@ -489,15 +489,14 @@ __device__ __forceinline__ T2 apply_kop(const KOp &op, int64_t k, int64_t N, T2 
 // is formed in FP64 (k'^2 exact up to 2^53, one rounding in the fma), split
 // as u = j + f with j integer and |f| <= 1/2; exp(2 pi i j / 256) comes from
 // a 256-entry table held in LDS and exp(2 pi i f / 256) from its Taylor
-// terms to third (sine) and fourth (cosine) order, truncation < 3e-12 and
-// 2e-15 for |2 pi f / 256| <= 1.23e-2.  Table entries and products are
-// complex64.
-constexpr int kCisTab = 256;
+// terms to third (sine) and second (cosine) order, truncation < 2.4e-12 and
+// 9.5e-10 for |2 pi f / 256| <= 1.23e-2, both below complex64 rounding.
+// Table entries and products are complex64.
 
 struct KopF32 {
   const float2 *tab;     // kCisTab entries exp(2 pi i j / kCisTab) (LDS)
   double au, bu;         // alpha, beta in 1/kCisTab revolutions
-  float s1, s3, c2, c4, sc;  // sc*w, -sc*w^3/6, -sc*w^2/2, sc*w^4/24, sc (w = phi per unit of f)
+  float s1, s3, c2, sc;  // sc*w, -sc*w^3/6, -sc*w^2/2, sc (w = phi per unit of f; KOp::t_*)
   int rs, rlen;          // cutoff band [rs, rs + rlen)
 };
 
@@ -506,13 +505,10 @@ __device__ __forceinline__ KopF32 kop_f32_setup(const KOp &op, const float2 *tab
   p.tab = tab;
   p.au = op.alpha_rev * (double)kCisTab;
   p.bu = op.beta_rev * (double)kCisTab;
-  const double w = 6.283185307179586 / (double)kCisTab;
-  const double sc = op.scale[0];
-  p.sc = (float)sc;
-  p.s1 = (float)(sc * w);
-  p.s3 = (float)(-sc * w * w * w / 6.0);
-  p.c2 = (float)(-sc * w * w / 2.0);
-  p.c4 = (float)(sc * w * w * w * w / 24.0);
+  p.sc = op.t_sc;
+  p.s1 = op.t_s1;
+  p.s3 = op.t_s3;
+  p.c2 = op.t_c2;
   p.rs = (int)op.rs;
   p.rlen = (int)(op.re - op.rs);
   return p;
@@ -546,8 +542,10 @@ template <bool CUT> __device__ __forceinline__ float2 kop_f32(const KopF32 &p, d
   // j mod kCisTab from the low mantissa bits of j + 1.5 * 2^52 (|j| < 2^51)
   const int idx = __double2loint(j + 6755399441055744.0) & (kCisTab - 1);
   const float2 T = p.tab[idx];
+  // |w f| <= pi / 256: the cosine's next term, (w f)^4 / 24 < 1e-9, is below
+  // complex64 rounding; the sine's cubic term (< 3.1e-7) is not
   const float f2 = f * f;
-  const float cs = fmaf(f2, fmaf(f2, p.c4, p.c2), p.sc);
+  const float cs = fmaf(f2, p.c2, p.sc);
   const float sn = f * fmaf(f2, p.s3, p.s1);
   float2 h;
   h.x = T.x * cs - T.y * sn;

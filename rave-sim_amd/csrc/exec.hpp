@@ -77,6 +77,12 @@ inline KOp make_kop(const PropAcc &a, int64_t N, bool fold_scale) {
   k.re = a.cut ? a.re : 0;
   k.scale[0] = fold_scale ? 1.0 / (double)N : 1.0;
   k.scale[1] = 0.0;
+  const double w = 6.283185307179586 / (double)kCisTab;
+  const double sc = k.scale[0];
+  k.t_sc = (float)sc;
+  k.t_s1 = (float)(sc * w);
+  k.t_s3 = (float)(-sc * w * w * w / 6.0);
+  k.t_c2 = (float)(-sc * w * w / 2.0);
   return k;
 }
 

@@ -65,6 +65,9 @@ struct XOp {
 // grating mask or scalar with the factors folded on the host, or any kind
 enum XopVariant : int32_t { XV_NONE = 0, XV_MASK = 1, XV_GEN = 2 };
 
+// entries of the complex64 row pass's phase table exp(2 pi i j / kCisTab)
+constexpr int kCisTab = 256;
+
 // spectrum multiply in the row pass: exp(2 pi i (alpha k'^2 + beta)) * scale,
 // zero for k in [rs, re) (big-wave/propagation.py:74-138).
 struct KOp {
@@ -73,6 +76,10 @@ struct KOp {
   double alpha_rev, beta_rev;
   int64_t rs, re;
   double scale[2];
+  // complex64 row passes: Taylor coefficients of exp(i w f) * scale[0] with
+  // w = 2 pi / 256 (phase per unit of the table residual f), formed on the
+  // host: sc * w, -sc * w^3 / 6, -sc * w^2 / 2, sc
+  float t_s1, t_s3, t_c2, t_sc;
 };
 
 struct FieldArgs {
