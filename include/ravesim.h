@@ -336,7 +336,9 @@ int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, d
  * this way. */
 int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA);
 
-/* Enable (1) / disable (0) per-launch event timing and reset the counters. */
+/* Per-launch event timing: 0 = off, 1 = every launch, k > 1 = every k-th
+ * launch of each kernel class (a sample; each timed launch costs a few
+ * microseconds).  Resets the counters. */
 int rs_set_timing(rs_ctx *ctx, int enable);
 /* Copy min(n, 4) class records. */
 int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n);

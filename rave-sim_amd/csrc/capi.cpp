@@ -612,8 +612,11 @@ int rs_server_destroy(rs_server *srv) {
 
 int rs_set_timing(rs_ctx *ctx, int enable) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (enable < 0) return fail(RS_ERR_INVALID, "timing stride must be >= 0");
   ctx->ctx->timing = enable != 0;
+  ctx->ctx->timing_stride = enable > 0 ? enable : 1;
   for (auto &s : ctx->ctx->stats) s = rs::PassStat{};
+  for (auto &n : ctx->ctx->timing_seen) n = 0;
   return RS_OK;
 }
 

@@ -174,6 +174,8 @@ class Context {
   // and joined into `stream` at the end of each run)
   hipStream_t lane_stream(int i);
   bool timing = false;
+  int timing_stride = 1;              // time every timing_stride-th launch of a class
+  int64_t timing_seen[PC_COUNT] = {};  // launches of each class since timing was set
   PassStat stats[PC_COUNT];
   // plan for (N, precision); explicit lgR/lgA, else the split tuned for the
   // nearest batch size (fields per run), else the default split

@@ -504,22 +504,31 @@ template <typename S> struct Exec {
       }
     };
 
-    // optional per-launch event timing (not combined with graph capture)
+    // optional per-launch event timing (not combined with graph capture).
+    // Timing only: the events carry no system-scope fence (cache write-back)
+    // and are read after the stream has been synchronised.  Fiber passes
+    // hand them to the launch, whose dispatch records the kernel's own start
+    // and end; other kernels get event records around the launch.
     const bool timed = ctx.timing && !rc.use_graph;
-    auto t_begin = [&](int cls, double bytes) {
-      if (!timed) return;
+    auto sampled = [&](int cls) { return timed && ctx.timing_seen[cls]++ % ctx.timing_stride == 0; };
+    auto t_new = [&](int cls, double bytes) {
       hipEvent_t a, b;
-      // timing only: no system-scope fence (cache write-back) at each record;
-      // the times are read after the stream has been synchronised
       hip_check(hipEventCreateWithFlags(&a, hipEventDisableSystemFence), "event");
       hip_check(hipEventCreateWithFlags(&b, hipEventDisableSystemFence), "event");
       ev.push_back(a);
       ev.push_back(b);
       ev_meta.emplace_back(cls, bytes);
-      hip_check(hipEventRecord(a, st), "event record");
+    };
+    bool open = false;
+    auto t_begin = [&](int cls, double bytes) {
+      open = sampled(cls);
+      if (!open) return;
+      t_new(cls, bytes);
+      hip_check(hipEventRecord(ev[ev.size() - 2], st), "event record");
     };
     auto t_end = [&]() {
-      if (timed) hip_check(hipEventRecord(ev.back(), st), "event record");
+      if (open) hip_check(hipEventRecord(ev.back(), st), "event record");
+      open = false;
     };
 
     for (const Launch &l : launches) {
@@ -528,9 +537,14 @@ template <typename S> struct Exec {
           begin_capture();
           const int cls = l.row ? PC_ROW : (l.geom.top ? PC_TOP : PC_MID);
           const double ns = (double)(l.geom.span_len ? l.geom.span_len : rc.N);
-          t_begin(cls, 2.0 * ns * (double)esz * l.nf);
-          hip_check(launch_fiber_pass<S>(l.lgL, l.row, l.xv, fa + l.arg_off, l.nf, l.geom, st), "fiber pass");
-          t_end();
+          hipEvent_t e0 = nullptr, e1 = nullptr;
+          if (sampled(cls)) {
+            t_new(cls, 2.0 * ns * (double)esz * l.nf);
+            e0 = ev[ev.size() - 2];
+            e1 = ev.back();
+          }
+          hip_check(launch_fiber_pass<S>(l.lgL, l.row, l.xv, fa + l.arg_off, l.nf, l.geom, st, e0, e1),
+                    "fiber pass");
           break;
         }
         case LKind::Xop:

@@ -10,6 +10,8 @@
 // The instantiations are spread over several translation units
 // (fiber_inst.hip built once per precision and variant).
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "device_ops.hpp"
 #include "rs_kernels.hpp"
 
@@ -272,14 +274,15 @@ static inline bool span_tiles(const Geom &g, int64_t per, Geom &gs, int64_t &nbl
 // columns per workgroup for a column pass of fiber length L: 16 columns of
 // complex64 (128 B per row segment) / 8 of complex128, more for short fibers
 template <typename S, int L, int XV>
-static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
+static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                             hipEvent_t t1) {
   constexpr int C = col_fibers(L, sizeof(S) == 8);
   if (g.S % C != 0) return hipErrorInvalidValue;
   Geom gs;
   int64_t nblk;
   if (!span_tiles(g, (int64_t)L * C, gs, nblk)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk, (unsigned)nf);
-  hipLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, args, gs);
+  hipExtLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0, args, gs);
   return hipGetLastError();
 }
 
@@ -287,7 +290,8 @@ static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipSt
 // (complex128) for rows up to 4096 / 2048 samples, one row beyond; fields
 // with fewer rows than that (single-group fields) use one row per workgroup
 template <typename S, int L, int XV>
-static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
+static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                             hipEvent_t t1) {
   constexpr int SPAN = sizeof(S) == 8 ? 2048 : 4096;
   constexpr int C = L >= SPAN ? 1 : SPAN / L;
   const int64_t rows = g.N / L;
@@ -295,42 +299,43 @@ static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipSt
   int64_t nblk;
   if (rows % C == 0 && span_tiles(g, (int64_t)L * C, gs, nblk)) {
     dim3 grid((unsigned)nblk, (unsigned)nf);
-    hipLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, args, gs);
+    hipExtLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0, args, gs);
   } else {
     if (!span_tiles(g, (int64_t)L, gs, nblk)) return hipErrorInvalidValue;
     dim3 grid((unsigned)nblk, (unsigned)nf);
-    hipLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, args, gs);
+    hipExtLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, t0, t1, 0, args, gs);
   }
   return hipGetLastError();
 }
 
 template <typename S, int XV>
-hipError_t launch_fiber_xv(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g, hipStream_t st) {
+hipError_t launch_fiber_xv(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g, hipStream_t st,
+                           hipEvent_t t0, hipEvent_t t1) {
   if (row) {
     switch (lgL) {
-      case 6: return launch_row<S, 64, XV>(args, nf, g, st);
-      case 7: return launch_row<S, 128, XV>(args, nf, g, st);
-      case 8: return launch_row<S, 256, XV>(args, nf, g, st);
-      case 9: return launch_row<S, 512, XV>(args, nf, g, st);
-      case 10: return launch_row<S, 1024, XV>(args, nf, g, st);
-      case 11: return launch_row<S, 2048, XV>(args, nf, g, st);
-      case 12: return launch_row<S, 4096, XV>(args, nf, g, st);
-      case 13: if constexpr (sizeof(S) == 4) return launch_row<S, 8192, XV>(args, nf, g, st);
+      case 6: return launch_row<S, 64, XV>(args, nf, g, st, t0, t1);
+      case 7: return launch_row<S, 128, XV>(args, nf, g, st, t0, t1);
+      case 8: return launch_row<S, 256, XV>(args, nf, g, st, t0, t1);
+      case 9: return launch_row<S, 512, XV>(args, nf, g, st, t0, t1);
+      case 10: return launch_row<S, 1024, XV>(args, nf, g, st, t0, t1);
+      case 11: return launch_row<S, 2048, XV>(args, nf, g, st, t0, t1);
+      case 12: return launch_row<S, 4096, XV>(args, nf, g, st, t0, t1);
+      case 13: if constexpr (sizeof(S) == 4) return launch_row<S, 8192, XV>(args, nf, g, st, t0, t1);
                else return hipErrorInvalidValue;
       default: return hipErrorInvalidValue;
     }
   }
   switch (lgL) {
-    case 4: return launch_col<S, 16, XV>(args, nf, g, st);
-    case 5: return launch_col<S, 32, XV>(args, nf, g, st);
-    case 6: return launch_col<S, 64, XV>(args, nf, g, st);
-    case 7: return launch_col<S, 128, XV>(args, nf, g, st);
-    case 8: return launch_col<S, 256, XV>(args, nf, g, st);
-    case 9: return launch_col<S, 512, XV>(args, nf, g, st);
-    case 10: return launch_col<S, 1024, XV>(args, nf, g, st);
-    case 11: return launch_col<S, 2048, XV>(args, nf, g, st);
-    case 12: return launch_col<S, 4096, XV>(args, nf, g, st);
-    case 13: if constexpr (sizeof(S) == 4) return launch_col<S, 8192, XV>(args, nf, g, st);
+    case 4: return launch_col<S, 16, XV>(args, nf, g, st, t0, t1);
+    case 5: return launch_col<S, 32, XV>(args, nf, g, st, t0, t1);
+    case 6: return launch_col<S, 64, XV>(args, nf, g, st, t0, t1);
+    case 7: return launch_col<S, 128, XV>(args, nf, g, st, t0, t1);
+    case 8: return launch_col<S, 256, XV>(args, nf, g, st, t0, t1);
+    case 9: return launch_col<S, 512, XV>(args, nf, g, st, t0, t1);
+    case 10: return launch_col<S, 1024, XV>(args, nf, g, st, t0, t1);
+    case 11: return launch_col<S, 2048, XV>(args, nf, g, st, t0, t1);
+    case 12: return launch_col<S, 4096, XV>(args, nf, g, st, t0, t1);
+    case 13: if constexpr (sizeof(S) == 4) return launch_col<S, 8192, XV>(args, nf, g, st, t0, t1);
              else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }

@@ -12,5 +12,5 @@
 
 namespace rs {
 template hipError_t launch_fiber_xv<RS_FIBER_S, RS_FIBER_XV>(int, bool, const FieldArgs *, int, const Geom &,
-                                                             hipStream_t);
+                                                             hipStream_t, hipEvent_t, hipEvent_t);
 }  // namespace rs

@@ -24,17 +24,17 @@ namespace rs {
 
 // the fiber-pass variants are compiled in their own translation units
 extern template hipError_t launch_fiber_xv<float, XV_NONE>(int, bool, const FieldArgs *, int, const Geom &,
-                                                           hipStream_t);
+                                                           hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_fiber_xv<float, XV_MASK>(int, bool, const FieldArgs *, int, const Geom &,
-                                                           hipStream_t);
+                                                           hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_fiber_xv<float, XV_GEN>(int, bool, const FieldArgs *, int, const Geom &,
-                                                          hipStream_t);
+                                                          hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_fiber_xv<double, XV_NONE>(int, bool, const FieldArgs *, int, const Geom &,
-                                                            hipStream_t);
+                                                            hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_fiber_xv<double, XV_MASK>(int, bool, const FieldArgs *, int, const Geom &,
-                                                            hipStream_t);
+                                                            hipStream_t, hipEvent_t, hipEvent_t);
 extern template hipError_t launch_fiber_xv<double, XV_GEN>(int, bool, const FieldArgs *, int, const Geom &,
-                                                           hipStream_t);
+                                                           hipStream_t, hipEvent_t, hipEvent_t);
 
 // ---------------------------------------------------------------------------
 // plain-form elementwise passes
@@ -128,11 +128,11 @@ inline unsigned grid_stride_blocks(int64_t n) {
 
 template <typename S>
 hipError_t launch_fiber_pass(int lgL, bool row, int xv, const FieldArgs *args, int nf, const Geom &g,
-                             hipStream_t st) {
+                             hipStream_t st, hipEvent_t t0, hipEvent_t t1) {
   switch (xv) {
-    case XV_NONE: return launch_fiber_xv<S, XV_NONE>(lgL, row, args, nf, g, st);
-    case XV_MASK: return launch_fiber_xv<S, XV_MASK>(lgL, row, args, nf, g, st);
-    case XV_GEN: return launch_fiber_xv<S, XV_GEN>(lgL, row, args, nf, g, st);
+    case XV_NONE: return launch_fiber_xv<S, XV_NONE>(lgL, row, args, nf, g, st, t0, t1);
+    case XV_MASK: return launch_fiber_xv<S, XV_MASK>(lgL, row, args, nf, g, st, t0, t1);
+    case XV_GEN: return launch_fiber_xv<S, XV_GEN>(lgL, row, args, nf, g, st, t0, t1);
     default: return hipErrorInvalidValue;
   }
 }
@@ -175,8 +175,10 @@ hipError_t launch_copy(void *dst, const void *src, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-template hipError_t launch_fiber_pass<float>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t);
-template hipError_t launch_fiber_pass<double>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t);
+template hipError_t launch_fiber_pass<float>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t,
+                                             hipEvent_t, hipEvent_t);
+template hipError_t launch_fiber_pass<double>(int, bool, int, const FieldArgs *, int, const Geom &, hipStream_t,
+                                              hipEvent_t, hipEvent_t);
 template hipError_t launch_mask_fill<float>(int, const XOp &, const Geom &, void *, hipStream_t);
 template hipError_t launch_mask_fill<double>(int, const XOp &, const Geom &, void *, hipStream_t);
 template hipError_t launch_xop<float>(const FieldArgs *, int, int64_t, hipStream_t);

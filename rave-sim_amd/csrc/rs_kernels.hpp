@@ -11,13 +11,15 @@ namespace rs {
 // One digit-group pass over `nf` fields (blockIdx.y = field).  `row` selects
 // the contiguous lowest group; otherwise a strided column group of stride g.S.
 // xv (XopVariant) must cover the x-space ops of every field of the launch
-// (xop_variant below).
+// (xop_variant below).  Non-null t0 / t1 receive the kernel's own start and
+// end times (recorded by the dispatch itself, no extra stream commands).
 template <typename S>
 hipError_t launch_fiber_pass(int lgL, bool row, int xv, const FieldArgs *args, int nf, const Geom &g,
-                             hipStream_t st);
+                             hipStream_t st, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 // the same for one compile-time variant (fiber_inst.hip)
 template <typename S, int XV>
-hipError_t launch_fiber_xv(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g, hipStream_t st);
+hipError_t launch_fiber_xv(int lgL, bool row, const FieldArgs *args, int nf, const Geom &g, hipStream_t st,
+                           hipEvent_t t0, hipEvent_t t1);
 
 // Smallest variant that runs the x-space ops of host records a[0..nf): none,
 // grating mask / scalar (single chunk), or any kind.

@@ -38,9 +38,12 @@ class Context:
     def synchronize(self) -> None:
         check(lib.rs_ctx_synchronize(self.handle))
 
-    def set_timing(self, enable: bool) -> None:
-        """Per-launch HIP-event timing of the fused passes (resets counters)."""
-        check(lib.rs_set_timing(self.handle, 1 if enable else 0))
+    def set_timing(self, enable: bool, stride: int = 1) -> None:
+        """Per-launch HIP-event timing of the fused passes (resets counters):
+        every `stride`-th launch of each kernel class."""
+        if stride < 1:
+            raise ValueError("stride must be >= 1")
+        check(lib.rs_set_timing(self.handle, int(stride) if enable else 0))
 
     def autotune(self, N: int, dtype=np.complex64, members: int = 1) -> dict:
         """Pick the fastest digit-group split for (N, dtype) on this device,

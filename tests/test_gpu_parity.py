@@ -249,6 +249,41 @@ def test_forced_splits_match_oracle(N, dtype, lgR, lgA):
     c.close()
 
 
+def test_pass_timing_samples_every_kth_launch():
+    """rs_set_timing(k) times launches 0, k, 2k, ... of each pass class; the
+    algorithmic bytes of a timed launch are one read and one write of the
+    samples it covers (half the field per middle or row launch over two
+    spans); timing does not change results."""
+    import ravesim
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 18, 2e-9
+    c = ravesim.Context(0)
+    c.set_split(N, np.complex64, 8, 6)
+    g = ravesim.Grating(pitch=1.3e-6, dc=(0.45, 0.45), z_start=0.0, thickness=8e-6, nr_steps=5,
+                        db_a=complex(2e-6, 3e-8))
+    ms = [BatchMember(wl=ravesim.energy_to_wavelength(30000.0), gratings=[g], cutoff_angles=[0.2] * 2,
+                      plane_wave=True)]
+    kw = dict(det_size=2e-4, det_px=1e-6, det_py=1.0, z_detector=0.05, dtype=np.complex64, spans=2)
+    base = run_batch(c, N, dx, ms, **kw)
+    counts = {}
+    for k in (1, 3):
+        c.set_timing(True, stride=k)
+        assert np.array_equal(run_batch(c, N, dx, ms, **kw), base)
+        st = c.pass_stats()
+        c.set_timing(False)
+        counts[k] = {cls: st[cls][0] for cls in ("row", "top", "mid")}
+        for cls in ("row", "top", "mid"):
+            n, ms_, byts = st[cls]
+            assert n > 0 and ms_ > 0
+            per = N * 8 * 2 // (1 if cls == "top" else 2)
+            assert byts == n * per, cls
+    for cls in ("row", "top", "mid"):
+        assert counts[3][cls] == -(-counts[1][cls] // 3), cls
+    with pytest.raises(ValueError):
+        c.set_timing(True, stride=0)
+    c.close()
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_spans_give_identical_rows(dtype):
     """The middle, row and middle passes of each propagation run over 1..64
