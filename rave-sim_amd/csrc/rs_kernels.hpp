@@ -75,13 +75,16 @@ constexpr int small_tab_offset(int L, int NS) {
 }
 constexpr int small_tab_entries(int L) { return small_tab_offset(L, L); }
 
-// fibers per workgroup of a column pass of length L: 128 B row segments up to
-// L = 512 (more fibers for short L), then C * L fixed at 128 KB of LDS
+// fibers per workgroup of a column pass of length L: row segments of 256 B
+// (512 B for complex64 fibers up to 64) and at least one wavefront up to
+// L = 256; 128 B at L = 512, where 256 B would leave one workgroup per CU;
+// beyond that C * L fixed at 128 KB of LDS
 constexpr int col_fibers(int L, bool f64) {
   if (L >= 1024) return (f64 ? 8192 : 16384) / L;
-  if (L <= 16) return f64 ? 32 : 64;
-  if (L <= 64) return f64 ? 16 : 32;
-  return f64 ? 8 : 16;
+  if (L == 512) return f64 ? 8 : 16;
+  if (L <= 16) return 64;
+  if (f64) return L <= 32 ? 32 : 16;
+  return L <= 64 ? 64 : 32;
 }
 
 }  // namespace rs
