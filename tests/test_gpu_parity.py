@@ -380,6 +380,36 @@ def test_every_fiber_length(ctx, dtype, lgR, lgA):
     assert rel_l2(got, ref.u) < tol
 
 
+# every column length from 16 to 512 as the top group (grating mask inside
+# the pass) and as the middle group of a three-group plan, in both
+# precisions: each column tile width the passes use
+COL_CASES = ([(dt, 6, a, 0) for dt in (np.complex64, np.complex128) for a in range(4, 10)] +
+             [(dt, 6, 4, b) for dt in (np.complex64, np.complex128) for b in range(4, 10)])
+
+
+@pytest.mark.parametrize("dtype,lgR,lgA,lgB", COL_CASES)
+def test_every_column_tile(ctx, dtype, lgR, lgA, lgB):
+    import ravesim
+    N, dx = 2 ** (lgR + lgA + lgB), 2e-9
+    ctx.set_split(N, dtype, lgR, lgA)
+    wl = ravesim.energy_to_wavelength(30000.0)
+    gk = dict(pitch=0.37e-6, dc=(0.4, 0.55), z_start=0.0, thickness=6e-6, nr_steps=3, db_a=complex(2e-6, 4e-8))
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.upload(rand_field(N, 7 * lgA + lgB))
+    fc = 0.4 / dx
+    f.apply_grating(wl, ravesim.Grating(x_positions=[3.1e-8], **gk), cutoff_freq=fc)
+    f.propagate(wl, 1.3e-3, fc)
+    got = f.download()
+    f.close()
+    p = bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=1e-6, detector_pixel_size_x=1e-7,
+                     detector_pixel_size_y=1.0, wl=wl, chunk_size=N)
+    ref = bw.Fields(rand_field(N, 7 * lgA + lgB))
+    bw.grating_apply(ref, bw.Grating(x_positions=np.array([3.1e-8]), substrate_thickness=0.0, **gk), p, fc, 0)
+    bw.propagate(ref, dx, wl, 1.3e-3, N, fc)
+    tol = 1e-10 if dtype == np.complex128 else 2e-6
+    assert rel_l2(got, ref.u) < tol
+
+
 @pytest.mark.parametrize("dtype,tol", [("c16", 1e-10), ("c8", 1e-4)])
 def test_sample_element_matches_oracle(ctx, tmp_path, dtype, tol):
     """Sample grid rows (np.interp of delta + i beta between pixel centres,
