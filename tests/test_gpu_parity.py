@@ -285,6 +285,31 @@ def test_pass_timing_samples_every_kth_launch():
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_graph_capture_matches_eager(dtype):
+    """rs_batch_desc.use_graph captures each group's launch sequence in a
+    hipGraph and launches it: three graph runs, with eager runs and
+    synchronisations between them, each equal to the eager result (the same
+    kernels on the same inputs)."""
+    import ravesim
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 16, 2e-9
+    c = ravesim.Context(0)
+    g = ravesim.Grating(pitch=0.9e-6, dc=(0.45, 0.5), z_start=0.0, thickness=6e-6, nr_steps=3,
+                        x_positions=[2e-7], db_a=complex(2e-6, 3e-8))
+    ms = [BatchMember(wl=ravesim.energy_to_wavelength(e), gratings=[g], cutoff_angles=[0.2] * 2,
+                      source_x=1e-7 * k, source_z=-0.1, weight=0.5 + 0.1 * k)
+          for k, e in enumerate((25e3, 31e3, 37e3))]
+    kw = dict(det_size=5e-5, det_px=1e-6, det_py=1.0, z_detector=0.03, dtype=dtype, max_resident=1)
+    eager = run_batch(c, N, dx, ms, **kw)
+    for _ in range(3):
+        got = run_batch(c, N, dx, ms, use_graph=True, **kw)
+        c.synchronize()
+        assert np.array_equal(got, eager)
+        assert np.array_equal(run_batch(c, N, dx, ms, **kw), eager)
+    c.close()
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_spans_give_identical_rows(dtype):
     """The middle, row and middle passes of each propagation run over 1..64
     consecutive sub-ranges of the fields (a 2^26 complex64 field runs over two
