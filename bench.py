@@ -198,6 +198,45 @@ def rocfft_comparison_child(N):
     return json.loads(lines[-1])
 
 
+def simdir_end_to_end(ravesim, ctx, N, slices=64, reps=2):
+    """The cfg2 chain through the sim-dir boundary (`run_single_simulation`,
+    the fastwave / big-wave entry point) with a vector source of ones read
+    from a complex64 .npy: YAML parsing, the .npy read, the upload over
+    PCIe, the simulation and the detector write, by wall clock."""
+    import tempfile
+
+    import numpy as np
+    from ravesim.multisim import run_single_simulation
+    from ravesim.simdir import write_simdir
+    h, p, dx = 60e-6, 4.2e-6, 2e-9
+
+    def grating(z, mat):
+        return {"type": "grating", "pitch": p, "dc": [0.5, 0.5], "z_start": z, "thickness": h,
+                "nr_steps": slices, "x_positions": [0.0], "substrate_thickness": 0.0, "mat_a": mat,
+                "mat_b": None, "mat_substrate": None}
+
+    with tempfile.TemporaryDirectory(prefix="rs_simdir_") as tmp:
+        src = Path(tmp) / "ones.npy"
+        np.save(src, np.ones(N, dtype=np.complex64))
+        cfg = {"sim_params": {"N": N, "dx": dx, "z_detector": 0.4 + h, "detector_size": 1e-3,
+                              "detector_pixel_size_x": 1e-6, "detector_pixel_size_y": 1.0, "chunk_size": N},
+               "use_disk_vector": False, "save_final_u_vectors": False, "dtype": "c8",
+               "elements": [grating(0.0, ["Au", 19.32]), grating(0.2, ["Si", 2.34]), grating(0.4, ["Au", 19.32])]}
+        sub = {"energy": 46000.0, "source": {"type": "vector", "input_path": str(src), "z": 0.0},
+               "deltabeta_table": [[["Au", 19.32], [1.1e-6, 4.0e-8]], [["Si", 2.34], [2.6e-7, 1.0e-10]]]}
+        sd = write_simdir(Path(tmp) / "sim", cfg, [sub], [math.pi / 2] * 4)
+        run_single_simulation(sd, 0, ctx=ctx)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            rows = run_single_simulation(sd, 0, ctx=ctx)
+            times.append(time.perf_counter() - t0)
+    t = min(times)
+    return {"what": "cfg2 through run_single_simulation: YAML, 512 MiB complex64 vector source read and "
+                    "uploaded, 194 steps, detected.npy written", "ms_per_sim": round(t * 1e3, 2),
+            "sims_per_hour": round(3600.0 / t, 1), "detector_checksum": float(np.sum(rows))}
+
+
 def rocfft_comparison(N=2 ** 26, reps=20):
     """One propagation step FFT * propagator * iFFT on N complex64 samples with
     the vendor FFT (torch.fft -> hipFFT -> rocFFT), as a reference point for
@@ -343,6 +382,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparisons")
     ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
+    ap.add_argument("--no-simdir", action="store_true", help="skip the sim-dir (PCIe-inclusive) run")
     ap.add_argument("--serve-qps", type=float, default=1.0,
                     help="serving run arrival rate in requests/s (0 = skip); the mix saturates near 1.7/s")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
@@ -420,6 +460,13 @@ def main():
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
     batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx)
+    simdir = None
+    if not (args.no_simdir or args.plain):
+        try:
+            simdir = simdir_end_to_end(ravesim, ctx, N, args.slices)
+            simdir["vs_device_resident"] = round(simdir["sims_per_hour"] / sims_per_hour, 4)
+        except Exception as e:  # a comparison only: report why it is missing
+            simdir = {"error": repr(e)}
     mix = None
     if not (args.no_batch or args.plain) and args.mix_sims > 0:
         mix = uneven_mix(ravesim, ctx, args.mix_sims)
@@ -461,6 +508,7 @@ def main():
         "uneven_mix": mix,
         "vendor_fft": vendor,
         "serving": serving,
+        "simdir_end_to_end": simdir,
         "detector_checksum": float(np.sum(out)),
     }
     print(json.dumps(line), flush=True)

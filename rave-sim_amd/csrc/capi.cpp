@@ -166,31 +166,18 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
     std::vector<std::vector<char>> saved;
     const bool save_u = c.save_final_u_vectors && o.write_output;
     if (save_u) m.saved_u = &saved;
-    std::vector<char> src_host;
     if (c.source.kind == rs::SourceCfg::Vector) {
-      rs::NpyArray a = rs::npy_load(c.source.input_path);
-      if (a.shape.size() != 1 || a.shape[0] != c.N)
+      // checked here, streamed to the device in the run (Context::stream_npy)
+      const rs::NpyHeader h = rs::npy_header(c.source.input_path);
+      if (h.shape.size() != 1 || h.shape[0] != c.N)
         throw rs::Error(RS_ERR_CONFIG, "source vector must have shape (N,)");
-      const int64_t n = c.N;
-      src_host.resize((size_t)n * (c.f64 ? 16 : 8));
-      if (a.descr == "<c16") {
-        const double *p = (const double *)a.data.data();
-        if (c.f64) std::memcpy(src_host.data(), p, src_host.size());
-        else {
-          float *q = (float *)src_host.data();
-          for (int64_t i = 0; i < 2 * n; ++i) q[i] = (float)p[i];
-        }
-      } else if (a.descr == "<c8") {
-        const float *p = (const float *)a.data.data();
-        if (!c.f64) std::memcpy(src_host.data(), p, src_host.size());
-        else {
-          double *q = (double *)src_host.data();
-          for (int64_t i = 0; i < 2 * n; ++i) q[i] = (double)p[i];
-        }
-      } else {
+      if (h.descr != "<c16" && h.descr != "<c8")
         throw rs::Error(RS_ERR_CONFIG, "source vector must be complex64 or complex128");
-      }
-      m.upload = src_host.data();
+      const int64_t esz = h.descr == "<c16" ? 16 : 8;
+      if (h.file_bytes < h.data_offset + c.N * esz) throw rs::Error(3, c.source.input_path + ": truncated data");
+      m.upload_npy = c.source.input_path;
+      m.upload_offset = h.data_offset;
+      m.upload_c16 = h.descr == "<c16";
     }
     if (!(o.history_dz >= 0)) throw rs::Error(RS_ERR_INVALID, "history_dz must be >= 0");
     rs::Recording rec;

@@ -3,6 +3,8 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <memory>
+#include <cstdio>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -353,9 +355,51 @@ Context::Context(int dev) : device(dev) {
 }
 Context::~Context() {
   plans_.clear();
+  for (int b = 0; b < 2; ++b) {
+    if (stage_ev_[b]) {
+      (void)hipEventSynchronize(stage_ev_[b]);
+      (void)hipEventDestroy(stage_ev_[b]);
+    }
+    if (stage_[b]) (void)hipHostFree(stage_[b]);
+  }
   for (hipStream_t s : lanes_) (void)hipStreamDestroy(s);
   if (stream) (void)hipStreamDestroy(stream);
 }
+void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st) {
+  std::FILE *f = std::fopen(m.upload_npy.c_str(), "rb");
+  if (!f) throw Error(3, "cannot open " + m.upload_npy);
+  std::unique_ptr<std::FILE, int (*)(std::FILE *)> close(f, std::fclose);
+  if (fseeko(f, (off_t)m.upload_offset, SEEK_SET) != 0) throw Error(3, m.upload_npy + ": seek failed");
+  for (int b = 0; b < 2; ++b) {
+    if (!stage_[b]) hip_check(hipHostMalloc(&stage_[b], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
+    if (!stage_ev_[b]) hip_check(hipEventCreateWithFlags(&stage_ev_[b], hipEventDisableTiming), "event");
+  }
+  const size_t desz = f64 ? 16 : 8, sesz = m.upload_c16 ? 16 : 8;
+  const int64_t per = (int64_t)(kStageBytes / desz);  // samples per chunk
+  std::vector<char> conv(sesz != desz ? (size_t)per * sesz : 0);
+  for (int64_t off = 0, k = 0; off < N; off += per, ++k) {
+    const int b = (int)(k & 1);
+    const int64_t n = std::min(per, N - off);
+    if (stage_used_[b]) hip_check(hipEventSynchronize(stage_ev_[b]), "staging wait");
+    char *buf = (char *)stage_[b];
+    char *rd = sesz == desz ? buf : conv.data();
+    if (std::fread(rd, sesz, (size_t)n, f) != (size_t)n) throw Error(3, m.upload_npy + ": truncated data");
+    if (sesz == 16 && desz == 8) {
+      const double *s = (const double *)rd;
+      float *d = (float *)buf;
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
+    } else if (sesz == 8 && desz == 16) {
+      const float *s = (const float *)rd;
+      double *d = (double *)buf;
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+    }
+    hip_check(hipMemcpyAsync((char *)dst + (size_t)off * desz, buf, (size_t)n * desz, hipMemcpyHostToDevice, st),
+              "upload");
+    hip_check(hipEventRecord(stage_ev_[b], st), "event record");
+    stage_used_[b] = true;
+  }
+}
+
 hipStream_t Context::lane_stream(int i) {
   if (i == 0) return stream;
   while ((int)lanes_.size() < i) {

@@ -126,6 +126,11 @@ struct Member {
   std::vector<Op> ops;
   std::vector<SampleRows> samples;  // device copies are made per run
   const void *upload = nullptr; // host data for the Upload op (natural order)
+  // or, when upload is null, a .npy vector streamed from disk through the
+  // context's pinned staging buffers (no host copy of the whole field)
+  std::string upload_npy;       // path
+  int64_t upload_offset = 0;    // byte offset of the data in the file
+  bool upload_c16 = false;      // file elements are complex128 (else complex64)
   std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
   std::vector<std::vector<char>> *keypoints = nullptr;  // Keypoint sink (indexed by the op's row)
 };
@@ -192,8 +197,17 @@ class Context {
   };
   std::vector<Tuned> tuned;
   const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
+  // Copy N natural-order samples of a .npy vector (member m's upload_npy)
+  // to the device field dst on stream st: chunks are read into two pinned
+  // staging buffers in turn, each reused once its previous copy has left it,
+  // so file reads overlap the transfers.
+  void stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st);
 
  private:
+  static constexpr size_t kStageBytes = size_t(16) << 20;
+  void *stage_[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev_[2] = {nullptr, nullptr};
+  bool stage_used_[2] = {false, false};
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;

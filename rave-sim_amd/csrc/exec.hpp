@@ -14,7 +14,7 @@
 namespace rs {
 namespace exec {
 
-enum class LKind { Fiber, Xop, Kop, Detect, Copy, Fill, SaveU };
+enum class LKind { Fiber, Xop, Kop, Detect, Copy, Fill, SaveU, Upload };
 
 struct Launch {
   LKind kind;
@@ -348,11 +348,10 @@ template <typename S> struct Exec {
           break;
         case OpKind::Upload: {
           for (int i = 0; i < K; ++i) {
-            Launch l{LKind::Copy};
+            Launch l{LKind::Upload};
             l.dst = field[i];
-            l.src = mem[i]->upload;
             l.n = rc.N;
-            l.member = -1;  // host to device
+            l.member = i;
             launches.push_back(l);
           }
           clear_px();
@@ -564,14 +563,18 @@ template <typename S> struct Exec {
           hip_check(launch_detect<S>(da + l.arg_off, l.nf, bounds, npx, st), "detect");
           break;
         case LKind::Copy:
-          if (l.member == -1) {
-            end_capture();
-            hip_check(hipMemcpyAsync(l.dst, l.src, l.n * esz, hipMemcpyHostToDevice, st), "upload");
-          } else {
-            begin_capture();
-            hip_check(launch_copy<S>(l.dst, l.src, l.n, st), "copy");
-          }
+          begin_capture();
+          hip_check(launch_copy<S>(l.dst, l.src, l.n, st), "copy");
           break;
+        case LKind::Upload: {
+          end_capture();
+          const Member &m = *mem[l.member];
+          if (m.upload)
+            hip_check(hipMemcpyAsync(l.dst, m.upload, l.n * esz, hipMemcpyHostToDevice, st), "upload");
+          else
+            ctx.stream_npy(m, sizeof(S) == 8, l.n, l.dst, st);
+          break;
+        }
         case LKind::Fill:
           begin_capture();
           hip_check(launch_fill_f64((double *)l.dst, l.n, 0.0, st), "fill");

@@ -30,9 +30,9 @@ std::string dict_value(const std::string &h, const std::string &key) {
 }
 }  // namespace
 
-NpyArray npy_load(const std::string &path) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f) throw Error(3, "cannot open " + path);
+namespace {
+// reads the header of the open file f; leaves f at the first data byte
+NpyHeader read_header(std::ifstream &f, const std::string &path) {
   char magic[8];
   f.read(magic, 8);
   if (!f || std::memcmp(magic, "\x93NUMPY", 6) != 0) throw Error(3, path + " is not a .npy file");
@@ -49,7 +49,8 @@ NpyArray npy_load(const std::string &path) {
   }
   std::string h(hlen, ' ');
   f.read(&h[0], hlen);
-  NpyArray a;
+  if (!f) throw Error(3, path + ": truncated header");
+  NpyHeader a;
   a.descr = dict_value(h, "descr");
   if (dict_value(h, "fortran_order").find("True") != std::string::npos)
     throw Error(3, path + ": Fortran-ordered arrays are not supported");
@@ -62,16 +63,36 @@ NpyArray npy_load(const std::string &path) {
       num.clear();
     }
   }
-  int64_t isz = 0;
-  const std::string d = a.descr;
-  if (d == "<c16") isz = 16;
-  else if (d == "<c8") isz = 8;
-  else if (d == "<f8") isz = 8;
-  else if (d == "<f4") isz = 4;
-  else if (d == "|i1" || d == "|u1" || d == "<i1") isz = 1;
-  else if (d == "<i8") isz = 8;
-  else throw Error(3, path + ": unsupported dtype " + d);
-  a.data.resize((size_t)(a.count() * isz));
+  a.data_offset = (int64_t)f.tellg();
+  return a;
+}
+
+int64_t item_size(const std::string &d, const std::string &path) {
+  if (d == "<c16") return 16;
+  if (d == "<c8" || d == "<f8" || d == "<i8") return 8;
+  if (d == "<f4") return 4;
+  if (d == "|i1" || d == "|u1" || d == "<i1") return 1;
+  throw Error(3, path + ": unsupported dtype " + d);
+}
+}  // namespace
+
+NpyHeader npy_header(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(3, "cannot open " + path);
+  NpyHeader h = read_header(f, path);
+  f.seekg(0, std::ios::end);
+  h.file_bytes = (int64_t)f.tellg();
+  return h;
+}
+
+NpyArray npy_load(const std::string &path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw Error(3, "cannot open " + path);
+  const NpyHeader h = read_header(f, path);
+  NpyArray a;
+  a.descr = h.descr;
+  a.shape = h.shape;
+  a.data.resize((size_t)(a.count() * item_size(a.descr, path)));
   f.read(a.data.data(), (std::streamsize)a.data.size());
   if (!f) throw Error(3, path + ": truncated data");
   return a;

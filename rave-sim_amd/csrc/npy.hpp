@@ -18,6 +18,16 @@ struct NpyArray {
   }
 };
 
+// header of a .npy file: element type, shape, where the data starts and the
+// file's size in bytes
+struct NpyHeader {
+  std::string descr;
+  std::vector<int64_t> shape;
+  int64_t data_offset = 0;
+  int64_t file_bytes = 0;
+};
+
+NpyHeader npy_header(const std::string &path);
 NpyArray npy_load(const std::string &path);
 void npy_save(const std::string &path, const std::string &descr, const std::vector<int64_t> &shape,
               const void *data, size_t bytes);

@@ -184,6 +184,33 @@ def test_cfg1_reduced_end_to_end(ctx, tmp_path, dtype):
     assert rel_l2(got, ref) < tol
 
 
+@pytest.mark.parametrize("N,file_dtype,dtype", [(2 ** 16, np.complex64, "c8"), (2 ** 16, np.complex128, "c8"),
+                                                (2 ** 22, np.complex64, "c16"), (2 ** 21, np.complex128, "c16")])
+def test_vector_source_streams_from_npy(ctx, tmp_path, N, file_dtype, dtype):
+    """A vector source is read from its .npy in chunks through pinned staging
+    buffers (several chunks and both buffers at 2^21 / 2^22, with conversion
+    between complex64 files and complex128 fields and back): the detector rows
+    match the oracle, which loads the whole file; a truncated file is an
+    error before anything runs."""
+    from ravesim import _lib
+    from ravesim.multisim import run_single_simulation
+    cfg, subs, ang = cfg1(tmp_path, N=N, slices=2, det_size=min(1e-4, N * 2e-9 / 2), dtype=dtype)
+    rng = np.random.default_rng(N)
+    u = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(file_dtype)
+    src = tmp_path / "src.npy"
+    np.save(src, u)
+    subs[0]["source"] = {"type": "vector", "input_path": str(src), "z": 0.0}
+    sd = _simdir(tmp_path, (cfg, subs, ang))
+    got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    tol = 1e-10 if dtype == "c16" else 1e-4
+    assert rel_l2(got, ref) < tol
+    raw = src.read_bytes()
+    src.write_bytes(raw[:-8])
+    with pytest.raises(_lib.RsError, match="truncated"):
+        run_single_simulation(sd, 0, tmp_path, ctx=ctx)
+
+
 def test_cfg2_reduced_phase_stepping(ctx, tmp_path):
     from ravesim.multisim import run_single_simulation
     sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 18, slices=6, det_size=1e-4, dtype="c8", g2_steps=8))
