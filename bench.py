@@ -418,7 +418,7 @@ def main():
         ctx.set_split(N, dtype, lgR, lgA)
         tuned = {"lgR": lgR, "lgA": lgA, "fixed": True}
     else:
-        tuned = None if args.plain or args.no_tune else ctx.autotune(N, dtype)
+        tuned = {"default": True} if args.plain or args.no_tune else ctx.autotune(N, dtype)
     for _ in range(args.warmup):
         run_batch(ctx, N, 2e-9, members, **kw)
     ctx.synchronize()
