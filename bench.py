@@ -30,7 +30,7 @@ TIMING_STRIDE = 7
 WL_46 = None
 
 
-def cfg2_members(ravesim, N, slices=64, plain=False):
+def talbot_lau_members(ravesim, N, slices=64, plain=False):
     wl = ravesim.energy_to_wavelength(46000.0)
     h, p = 60e-6, 4.2e-6
     g0 = ravesim.Grating(pitch=p, dc=(0.5, 0.5), z_start=0.0, thickness=h, nr_steps=slices,
@@ -44,7 +44,7 @@ def cfg2_members(ravesim, N, slices=64, plain=False):
     return [m], 0.4 + h, 3 * slices + 2
 
 
-def cfg3_members(ravesim, N, n_energies=64, slices=32):
+def poly_members(ravesim, N, n_energies=64, slices=32):
     """Polychromatic batch: 64 energies 20-70 keV on the single-grating chain
     (Si phase grating p 4.2 um, h 30 um, 32 slices, 0.1 m hop), weights of a
     70 kVp tungsten spectrum (Kramers (70-E)/E times 3 mm Al, mu ~ E^-3,
@@ -71,7 +71,7 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
     import numpy as np
     from ravesim.core import run_batch
     N = 2 ** lgN
-    ms, zdet, steps = cfg3_members(ravesim, N)
+    ms, zdet, steps = poly_members(ravesim, N)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet)
     tuned = ctx.autotune(N, np.complex64, members=len(ms))
     run_batch(ctx, N, 2e-9, ms[:2], dtype=np.complex64, **kw)
@@ -199,7 +199,7 @@ def rocfft_comparison_child(N):
 
 
 def simdir_end_to_end(ravesim, ctx, N, slices=64, reps=2):
-    """The cfg2 chain through the sim-dir boundary (`run_single_simulation`,
+    """The Talbot-Lau chain through the sim-dir boundary (`run_single_simulation`,
     the fastwave / big-wave entry point) with a vector source of ones read
     from a complex64 .npy: YAML parsing, the .npy read, the upload over
     PCIe, the simulation and the detector write, by wall clock."""
@@ -232,7 +232,7 @@ def simdir_end_to_end(ravesim, ctx, N, slices=64, reps=2):
             rows = run_single_simulation(sd, 0, ctx=ctx)
             times.append(time.perf_counter() - t0)
     t = min(times)
-    return {"what": "cfg2 through run_single_simulation: YAML, 512 MiB complex64 vector source read and "
+    return {"what": "Talbot-Lau chain (N=2^26) through run_single_simulation: YAML, 512 MiB complex64 vector source read and "
                     "uploaded, 194 steps, detected.npy written", "ms_per_sim": round(t * 1e3, 2),
             "sims_per_hour": round(3600.0 / t, 1), "detector_checksum": float(np.sum(rows))}
 
@@ -378,7 +378,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
     ap.add_argument("--split", default="", help="fixed digit-group split 'lgR,lgA' instead of tuning")
-    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_cfg2.json"))
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_talbot_lau_n26.json"))
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparisons")
     ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
@@ -409,7 +409,7 @@ def main():
 
     N = 2 ** args.lgN
     dtype = np.complex64 if args.dtype == "c8" else np.complex128
-    members, zdet, steps_per_sim = cfg2_members(ravesim, N, args.slices)
+    members, zdet, steps_per_sim = talbot_lau_members(ravesim, N, args.slices)
     ctx = ravesim.Context(0)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet, dtype=dtype, plain=args.plain)
 

@@ -35,11 +35,11 @@ def simdir_rows(case_fn, **kw):
         return bw.run_simdir(sd, 0, dtype=np.complex128)
 
 
-def cfg3_rows(n_energies, N, slices):
-    """Weighted polychromatic sum over the cfg-1 chain (plane wave)."""
+def poly_rows(n_energies, N, slices):
+    """Weighted polychromatic sum over the monochromatic chain (plane wave, one Si phase grating and a hop)."""
     dx, h = 2e-9, 30e-6
     total = None
-    for e_ev, w, db in simcases.cfg3_members(n_energies, N, slices):
+    for e_ev, w, db in simcases.poly_members(n_energies, N, slices):
         wl = bw.convert_energy_wavelength(e_ev)
         p = bw.SimParams(N=N, dx=dx, z_detector=h + 0.1, detector_size=1e-4, detector_pixel_size_x=1e-6,
                          detector_pixel_size_y=1.0, wl=wl, chunk_size=N)
@@ -53,12 +53,12 @@ def cfg3_rows(n_energies, N, slices):
 
 def main():
     fx = {}
-    fx["cfg1_n16"] = dict(rows=simdir_rows(simcases.cfg1, N=2 ** 16, slices=32, det_size=1e-4),
+    fx["mono_n16"] = dict(rows=simdir_rows(simcases.mono_chain, N=2 ** 16, slices=32, det_size=1e-4),
                           N=2 ** 16, slices=32, det_size=1e-4)
-    fx["cfg2_n18_steps8"] = dict(rows=simdir_rows(simcases.cfg2, N=2 ** 18, slices=8, det_size=1e-4,
+    fx["talbot_lau_n18_steps8"] = dict(rows=simdir_rows(simcases.talbot_lau, N=2 ** 18, slices=8, det_size=1e-4,
                                                   g2_steps=8),
                                  N=2 ** 18, slices=8, det_size=1e-4, g2_steps=8)
-    fx["cfg3_n16_e8"] = dict(rows=cfg3_rows(8, 2 ** 16, 16), N=2 ** 16, slices=16, n_energies=8)
+    fx["poly_n16_e8"] = dict(rows=poly_rows(8, 2 ** 16, 16), N=2 ** 16, slices=16, n_energies=8)
     for name, d in fx.items():
         np.savez(HERE / f"{name}.npz", **{k: np.asarray(v) for k, v in d.items()})
         print(name, d["rows"].shape)

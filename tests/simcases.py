@@ -40,7 +40,7 @@ def write_ones(path: Path, N: int) -> str:
     return str(path)
 
 
-def cfg1(tmp: Path, N=2 ** 20, slices=32, det_size=1e-3, dtype="c16", chunk=None):
+def mono_chain(tmp: Path, N=2 ** 20, slices=32, det_size=1e-3, dtype="c16", chunk=None):
     """Plain monochromatic chain: plane wave at 25 keV, one Si phase grating
     (p 4.2 um, h 30 um, 32 slices), 0.1 m hop, 1 um pixels."""
     dx = 2e-9
@@ -54,7 +54,7 @@ def cfg1(tmp: Path, N=2 ** 20, slices=32, det_size=1e-3, dtype="c16", chunk=None
     return cfg, subs, [math.pi / 2] * 2
 
 
-def cfg2(tmp: Path, N=2 ** 26, slices=64, det_size=None, dtype="c8", chunk=None, g2_steps=1):
+def talbot_lau(tmp: Path, N=2 ** 26, slices=64, det_size=None, dtype="c8", chunk=None, g2_steps=1):
     """Talbot-Lau 3-grating chain at 46 keV: G0 Au absorption, G1 Si phase,
     G2 = G0, 60 um high, 0.2 m start to start; plane wave at G0, detector at
     the end of G2.  g2_steps > 1 phase-steps G2 at k p / g2_steps."""
@@ -84,8 +84,8 @@ def spectrum_weights(energies_kev):
     return w / w.sum()
 
 
-def cfg3_members(n_energies=64, N=2 ** 24, slices=32):
-    """Polychromatic cfg-1 chain: energies 20..70 keV uniformly, delta ~ E^-2,
+def poly_members(n_energies=64, N=2 ** 24, slices=32):
+    """Polychromatic version of the monochromatic chain: energies 20..70 keV uniformly, delta ~ E^-2,
     beta ~ E^-4 scaled from the 25 keV values."""
     e = np.linspace(20.0, 70.0, n_energies)
     w = spectrum_weights(e)

@@ -29,20 +29,20 @@ def _simdir(tmp_path, case):
 
 
 @pytest.mark.parametrize("dtype,tol", [("c16", 1e-11), ("c8", 1e-4)])
-def test_cfg1_fixture(ctx, golden_dir, tmp_path, dtype, tol):
+def test_mono_fixture(ctx, golden_dir, tmp_path, dtype, tol):
     from ravesim.multisim import run_single_simulation
-    fx = np.load(golden_dir / "cfg1_n16.npz")
-    sd = _simdir(tmp_path, simcases.cfg1(tmp_path, N=int(fx["N"]), slices=int(fx["slices"]),
+    fx = np.load(golden_dir / "mono_n16.npz")
+    sd = _simdir(tmp_path, simcases.mono_chain(tmp_path, N=int(fx["N"]), slices=int(fx["slices"]),
                                          det_size=float(fx["det_size"]), dtype=dtype))
     got = run_single_simulation(sd, 0, ctx=ctx)
     assert rel_l2(got, fx["rows"]) < tol
 
 
 @pytest.mark.parametrize("dtype,tol", [("c16", 1e-11), ("c8", 1e-4)])
-def test_cfg2_phase_stepping_fixture(ctx, golden_dir, tmp_path, dtype, tol):
+def test_talbot_lau_phase_stepping_fixture(ctx, golden_dir, tmp_path, dtype, tol):
     from ravesim.multisim import run_single_simulation
-    fx = np.load(golden_dir / "cfg2_n18_steps8.npz")
-    sd = _simdir(tmp_path, simcases.cfg2(tmp_path, N=int(fx["N"]), slices=int(fx["slices"]),
+    fx = np.load(golden_dir / "talbot_lau_n18_steps8.npz")
+    sd = _simdir(tmp_path, simcases.talbot_lau(tmp_path, N=int(fx["N"]), slices=int(fx["slices"]),
                                          det_size=float(fx["det_size"]), dtype=dtype,
                                          g2_steps=int(fx["g2_steps"])))
     got = run_single_simulation(sd, 0, ctx=ctx)
@@ -51,14 +51,14 @@ def test_cfg2_phase_stepping_fixture(ctx, golden_dir, tmp_path, dtype, tol):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 1e-4)])
-def test_cfg3_polychromatic_batch_fixture(ctx, golden_dir, dtype, tol):
+def test_polychromatic_batch_fixture(ctx, golden_dir, dtype, tol):
     import ravesim
     from ravesim.core import BatchMember, run_batch
-    fx = np.load(golden_dir / "cfg3_n16_e8.npz")
+    fx = np.load(golden_dir / "poly_n16_e8.npz")
     N, slices = int(fx["N"]), int(fx["slices"])
     h = 30e-6
     members = []
-    for e_ev, w, db in simcases.cfg3_members(int(fx["n_energies"]), N, slices):
+    for e_ev, w, db in simcases.poly_members(int(fx["n_energies"]), N, slices):
         g = ravesim.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=h, nr_steps=slices,
                             db_a=db)
         members.append(BatchMember(wl=ravesim.energy_to_wavelength(e_ev), gratings=[g],

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import bigwave as bw
-from simcases import DB_SI_25KEV, cfg1, cfg2, sample_case
+from simcases import DB_SI_25KEV, mono_chain, talbot_lau, sample_case
 
 pytestmark = pytest.mark.gpu
 
@@ -172,9 +172,9 @@ def _simdir(tmp_path, case):
 
 
 @pytest.mark.parametrize("dtype", ["c16", "c8"])
-def test_cfg1_reduced_end_to_end(ctx, tmp_path, dtype):
+def test_mono_chain_end_to_end(ctx, tmp_path, dtype):
     from ravesim.multisim import run_single_simulation
-    sd = _simdir(tmp_path, cfg1(tmp_path, N=2 ** 16, slices=8, det_size=1e-4, dtype=dtype))
+    sd = _simdir(tmp_path, mono_chain(tmp_path, N=2 ** 16, slices=8, det_size=1e-4, dtype=dtype))
     got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     ref = bw.run_simdir(sd, 0, dtype=np.complex128)
     assert got.shape == ref.shape == (1, 100)
@@ -194,7 +194,7 @@ def test_vector_source_streams_from_npy(ctx, tmp_path, N, file_dtype, dtype):
     error before anything runs."""
     from ravesim import _lib
     from ravesim.multisim import run_single_simulation
-    cfg, subs, ang = cfg1(tmp_path, N=N, slices=2, det_size=min(1e-4, N * 2e-9 / 2), dtype=dtype)
+    cfg, subs, ang = mono_chain(tmp_path, N=N, slices=2, det_size=min(1e-4, N * 2e-9 / 2), dtype=dtype)
     rng = np.random.default_rng(N)
     u = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(file_dtype)
     src = tmp_path / "src.npy"
@@ -211,9 +211,9 @@ def test_vector_source_streams_from_npy(ctx, tmp_path, N, file_dtype, dtype):
         run_single_simulation(sd, 0, tmp_path, ctx=ctx)
 
 
-def test_cfg2_reduced_phase_stepping(ctx, tmp_path):
+def test_talbot_lau_phase_stepping(ctx, tmp_path):
     from ravesim.multisim import run_single_simulation
-    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 18, slices=6, det_size=1e-4, dtype="c8", g2_steps=8))
+    sd = _simdir(tmp_path, talbot_lau(tmp_path, N=2 ** 18, slices=6, det_size=1e-4, dtype="c8", g2_steps=8))
     got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     ref = bw.run_simdir(sd, 0, dtype=np.complex128)
     assert got.shape == ref.shape == (8, 100)
@@ -229,22 +229,22 @@ def test_cfg2_reduced_phase_stepping(ctx, tmp_path):
 
 def test_plain_and_fused_agree(ctx, tmp_path):
     from ravesim.multisim import run_single_simulation
-    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 17, slices=3, det_size=1e-4, dtype="c16", g2_steps=2))
+    sd = _simdir(tmp_path, talbot_lau(tmp_path, N=2 ** 17, slices=3, det_size=1e-4, dtype="c16", g2_steps=2))
     a = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     b = run_single_simulation(sd, 0, tmp_path, ctx=ctx, plain=True)
     assert rel_l2(a, b) < 1e-11
 
 
-def test_cfg2_full_step_count_error_budget(ctx, tmp_path):
+def test_talbot_lau_194_step_error_budget(ctx, tmp_path):
     """All 194 propagation steps of the Talbot-Lau chain (3 x 64 slices) at
     reduced N: the complex64 + FP64-phase form stays within 1e-4 relative L2
     of detector intensity against the complex128 oracle."""
     from ravesim.multisim import run_single_simulation
-    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 18, slices=64, det_size=1e-4, dtype="c8"))
+    sd = _simdir(tmp_path, talbot_lau(tmp_path, N=2 ** 18, slices=64, det_size=1e-4, dtype="c8"))
     got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     ref = bw.run_simdir(sd, 0, dtype=np.complex128)
     err = rel_l2(got, ref)
-    print(f"cfg2 N=2^18 194 steps c8 rel-L2 = {err:.3e}")
+    print(f"Talbot-Lau N=2^18 194 steps c8 rel-L2 = {err:.3e}")
     assert err < 1e-4
 
 
@@ -642,14 +642,14 @@ def test_fastwave_cli_matches_oracle(tmp_path):
     assert hist.shape == h.get_history().shape and rel_l2(hist, h.get_history()) < 1e-10
 
 
-def test_cfg2_full_size_precision_form(ctx, tmp_path):
+def test_talbot_lau_full_size_precision_form(ctx, tmp_path):
     """The precision form at the benchmark's full size: Talbot-Lau chain at
     N = 2^26 with G2 stepped over 8 positions (16 slices per grating).  Beyond
     the oracle's reach the reference is the plain complex128 GPU form (itself
     checked against the oracle at smaller N); the complex64 + FP64-phase form
     stays within 1e-4 relative L2 of intensity and 1e-3 rad of fringe phase."""
     from ravesim.multisim import run_single_simulation
-    sd = _simdir(tmp_path, cfg2(tmp_path, N=2 ** 26, slices=16, det_size=1e-3, dtype="c8", g2_steps=8))
+    sd = _simdir(tmp_path, talbot_lau(tmp_path, N=2 ** 26, slices=16, det_size=1e-3, dtype="c8", g2_steps=8))
     got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)
     ref = run_single_simulation(sd, 0, tmp_path, ctx=ctx, dtype="c16", plain=True)
     assert got.shape == ref.shape == (8, 1000)
@@ -659,7 +659,7 @@ def test_cfg2_full_size_precision_form(ctx, tmp_path):
     a, b = (w * got).sum(0), (w * ref).sum(0)
     ok = np.abs(b) > 1e-3 * np.abs(ref).mean() * 8
     dphi = np.abs(np.angle(a[ok] * np.conj(b[ok]))).max()
-    print(f"cfg2 N=2^26 c8 vs plain c16: rel-L2 {err:.3e}, fringe phase {dphi:.2e} rad over {ok.sum()} px")
+    print(f"Talbot-Lau N=2^26 c8 vs plain c16: rel-L2 {err:.3e}, fringe phase {dphi:.2e} rad over {ok.sum()} px")
     assert err < 1e-4 and dphi < 1e-3
 
 

@@ -11,7 +11,7 @@ import pytest
 import yaml
 
 from oracle import bigwave as bw
-from simcases import cfg1, cfg2
+from simcases import mono_chain, talbot_lau
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -75,12 +75,12 @@ def _write(tmp_path, case):
     return write_simdir(tmp_path / "sim", cfg, subs, ang)
 
 
-def test_validate_cfg1_and_cfg2(tmp_path):
+def test_validate_mono_and_talbot_lau(tmp_path):
     from ravesim.multisim import validate_simdir
-    sd = _write(tmp_path, cfg1(tmp_path, N=2 ** 16, slices=32, det_size=1e-4))
+    sd = _write(tmp_path, mono_chain(tmp_path, N=2 ** 16, slices=32, det_size=1e-4))
     N, bits, steps, npx, nel, props = validate_simdir(sd, 0)
     assert (N, bits, steps, npx, nel, props) == (2 ** 16, 16, 1, 100, 1, 33)
-    sd2 = _write(tmp_path / "b", cfg2(tmp_path, N=2 ** 18, slices=64, dtype="c8", g2_steps=8))
+    sd2 = _write(tmp_path / "b", talbot_lau(tmp_path, N=2 ** 18, slices=64, dtype="c8", g2_steps=8))
     N, bits, steps, npx, nel, props = validate_simdir(sd2, 0)
     assert (N, bits, steps, nel, props) == (2 ** 18, 8, 8, 3, 194)
 
@@ -154,7 +154,7 @@ deltabeta_table:
 def test_config_errors_are_reported(tmp_path, mutation, needle):
     from ravesim import _lib
     from ravesim.multisim import validate_simdir
-    cfg, subs, ang = cfg1(tmp_path, N=2 ** 12, slices=2, det_size=4e-6)
+    cfg, subs, ang = mono_chain(tmp_path, N=2 ** 12, slices=2, det_size=4e-6)
     mutation(cfg, subs)
     sd = _write(tmp_path, (cfg, subs, ang))
     with pytest.raises(_lib.RsError) as e:
@@ -308,7 +308,7 @@ def test_malformed_yaml_is_a_config_error(tmp_path):
     """A config.yaml that is not YAML is reported, not crashed on."""
     from ravesim import _lib
     from ravesim.multisim import validate_simdir
-    sd = _write(tmp_path, cfg1(tmp_path, N=2 ** 12, slices=2, det_size=4e-6))
+    sd = _write(tmp_path, mono_chain(tmp_path, N=2 ** 12, slices=2, det_size=4e-6))
     (sd / "config.yaml").write_text("sim_params: {N: 4096, dx: [1e-9\nelements: - : -\n\t]\n")
     with pytest.raises(_lib.RsError) as e:
         validate_simdir(sd, 0)

@@ -9,14 +9,14 @@ import make_fixtures  # noqa: E402
 import simcases  # noqa: E402
 
 
-def test_cfg1_fixture_reproduced(golden_dir):
-    fx = np.load(golden_dir / "cfg1_n16.npz")
-    rows = make_fixtures.simdir_rows(simcases.cfg1, N=int(fx["N"]), slices=int(fx["slices"]),
+def test_mono_fixture_reproduced(golden_dir):
+    fx = np.load(golden_dir / "mono_n16.npz")
+    rows = make_fixtures.simdir_rows(simcases.mono_chain, N=int(fx["N"]), slices=int(fx["slices"]),
                                      det_size=float(fx["det_size"]))
     assert np.array_equal(rows, fx["rows"])
 
 
-def test_cfg3_fixture_reproduced(golden_dir):
-    fx = np.load(golden_dir / "cfg3_n16_e8.npz")
-    rows = make_fixtures.cfg3_rows(int(fx["n_energies"]), int(fx["N"]), int(fx["slices"]))
+def test_poly_fixture_reproduced(golden_dir):
+    fx = np.load(golden_dir / "poly_n16_e8.npz")
+    rows = make_fixtures.poly_rows(int(fx["n_energies"]), int(fx["N"]), int(fx["slices"]))
     assert np.array_equal(rows, fx["rows"])
