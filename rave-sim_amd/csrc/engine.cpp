@@ -5,6 +5,11 @@
 #include <algorithm>
 #include <memory>
 #include <cstdio>
+#include <exception>
+#include <thread>
+
+#include <fcntl.h>
+#include <unistd.h>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -362,49 +367,79 @@ Context::Context(int dev) : device(dev) {
 }
 Context::~Context() {
   plans_.clear();
-  for (int b = 0; b < 2; ++b) {
-    if (stage_ev_[b]) {
-      (void)hipEventSynchronize(stage_ev_[b]);
-      (void)hipEventDestroy(stage_ev_[b]);
+  for (Stage &sg : stage_)
+    for (int b = 0; b < 2; ++b) {
+      if (sg.ev[b]) {
+        (void)hipEventSynchronize(sg.ev[b]);
+        (void)hipEventDestroy(sg.ev[b]);
+      }
+      if (sg.buf[b]) (void)hipHostFree(sg.buf[b]);
     }
-    if (stage_[b]) (void)hipHostFree(stage_[b]);
-  }
   for (hipStream_t s : lanes_) (void)hipStreamDestroy(s);
   if (stream) (void)hipStreamDestroy(stream);
 }
 void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st) {
-  std::FILE *f = std::fopen(m.upload_npy.c_str(), "rb");
-  if (!f) throw Error(3, "cannot open " + m.upload_npy);
-  std::unique_ptr<std::FILE, int (*)(std::FILE *)> close(f, std::fclose);
-  if (fseeko(f, (off_t)m.upload_offset, SEEK_SET) != 0) throw Error(3, m.upload_npy + ": seek failed");
-  for (int b = 0; b < 2; ++b) {
-    if (!stage_[b]) hip_check(hipHostMalloc(&stage_[b], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
-    if (!stage_ev_[b]) hip_check(hipEventCreateWithFlags(&stage_ev_[b], hipEventDisableTiming), "event");
-  }
+  const int fd = ::open(m.upload_npy.c_str(), O_RDONLY);
+  if (fd < 0) throw Error(3, "cannot open " + m.upload_npy);
+  struct Closer {
+    int fd;
+    ~Closer() { ::close(fd); }
+  } closer{fd};
   const size_t desz = f64 ? 16 : 8, sesz = m.upload_c16 ? 16 : 8;
   const int64_t per = (int64_t)(kStageBytes / desz);  // samples per chunk
-  std::vector<char> conv(sesz != desz ? (size_t)per * sesz : 0);
-  for (int64_t off = 0, k = 0; off < N; off += per, ++k) {
-    const int b = (int)(k & 1);
-    const int64_t n = std::min(per, N - off);
-    if (stage_used_[b]) hip_check(hipEventSynchronize(stage_ev_[b]), "staging wait");
-    char *buf = (char *)stage_[b];
-    char *rd = sesz == desz ? buf : conv.data();
-    if (std::fread(rd, sesz, (size_t)n, f) != (size_t)n) throw Error(3, m.upload_npy + ": truncated data");
-    if (sesz == 16 && desz == 8) {
-      const double *s = (const double *)rd;
-      float *d = (float *)buf;
-      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
-    } else if (sesz == 8 && desz == 16) {
-      const float *s = (const float *)rd;
-      double *d = (double *)buf;
-      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+  // one thread per 32 MiB of field, at most kStageThreads
+  const int T = (int)std::max<int64_t>(1, std::min<int64_t>(kStageThreads, (N * (int64_t)desz) >> 25));
+  for (int t = 0; t < T; ++t)
+    for (int b = 0; b < 2; ++b) {
+      if (!stage_[t].buf[b])
+        hip_check(hipHostMalloc(&stage_[t].buf[b], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
+      if (!stage_[t].ev[b]) hip_check(hipEventCreateWithFlags(&stage_[t].ev[b], hipEventDisableTiming), "event");
     }
-    hip_check(hipMemcpyAsync((char *)dst + (size_t)off * desz, buf, (size_t)n * desz, hipMemcpyHostToDevice, st),
-              "upload");
-    hip_check(hipEventRecord(stage_ev_[b], st), "event record");
-    stage_used_[b] = true;
-  }
+  std::vector<std::exception_ptr> err(T);
+  auto part = [&](int t) {
+    try {
+      hip_check(hipSetDevice(device), "hipSetDevice");
+      Stage &sg = stage_[t];
+      std::vector<char> conv(sesz != desz ? (size_t)per * sesz : 0);
+      const int64_t lo = N * t / T, hi = N * (t + 1) / T;
+      int k = 0;
+      for (int64_t off = lo; off < hi; off += per, ++k) {
+        const int b = k & 1;
+        const int64_t n = std::min(per, hi - off);
+        if (sg.used[b]) hip_check(hipEventSynchronize(sg.ev[b]), "staging wait");
+        char *buf = (char *)sg.buf[b];
+        char *rd = sesz == desz ? buf : conv.data();
+        size_t got = 0;
+        const size_t want = (size_t)n * sesz;
+        while (got < want) {
+          const ssize_t r = ::pread(fd, rd + got, want - got, (off_t)(m.upload_offset + off * (int64_t)sesz + got));
+          if (r <= 0) throw Error(3, m.upload_npy + ": truncated data");
+          got += (size_t)r;
+        }
+        if (sesz == 16 && desz == 8) {
+          const double *s = (const double *)rd;
+          float *d = (float *)buf;
+          for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
+        } else if (sesz == 8 && desz == 16) {
+          const float *s = (const float *)rd;
+          double *d = (double *)buf;
+          for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+        }
+        hip_check(hipMemcpyAsync((char *)dst + (size_t)off * desz, buf, (size_t)n * desz, hipMemcpyHostToDevice, st),
+                  "upload");
+        hip_check(hipEventRecord(sg.ev[b], st), "event record");
+        sg.used[b] = true;
+      }
+    } catch (...) {
+      err[t] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < T; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto &x : th) x.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 hipStream_t Context::lane_stream(int i) {

@@ -198,16 +198,21 @@ class Context {
   std::vector<Tuned> tuned;
   const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
   // Copy N natural-order samples of a .npy vector (member m's upload_npy)
-  // to the device field dst on stream st: chunks are read into two pinned
-  // staging buffers in turn, each reused once its previous copy has left it,
-  // so file reads overlap the transfers.
+  // to the device field dst on stream st.  Up to kStageThreads host threads
+  // each stream one contiguous part: chunks are read into the thread's two
+  // pinned staging buffers in turn, each reused once its previous copy has
+  // left it, so file reads overlap the transfers and each other.
   void stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st);
 
  private:
-  static constexpr size_t kStageBytes = size_t(16) << 20;
-  void *stage_[2] = {nullptr, nullptr};
-  hipEvent_t stage_ev_[2] = {nullptr, nullptr};
-  bool stage_used_[2] = {false, false};
+  static constexpr size_t kStageBytes = size_t(4) << 20;
+  static constexpr int kStageThreads = 4;
+  struct Stage {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+  };
+  Stage stage_[kStageThreads];
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;

@@ -185,10 +185,12 @@ def test_mono_chain_end_to_end(ctx, tmp_path, dtype):
 
 
 @pytest.mark.parametrize("N,file_dtype,dtype", [(2 ** 16, np.complex64, "c8"), (2 ** 16, np.complex128, "c8"),
-                                                (2 ** 22, np.complex64, "c16"), (2 ** 21, np.complex128, "c16")])
+                                                (2 ** 23, np.complex64, "c16"), (2 ** 21, np.complex128, "c16"),
+                                                (2 ** 24, np.complex64, "c8")])
 def test_vector_source_streams_from_npy(ctx, tmp_path, N, file_dtype, dtype):
     """A vector source is read from its .npy in chunks through pinned staging
-    buffers (several chunks and both buffers at 2^21 / 2^22, with conversion
+    buffers, by one host thread per 32 MiB of field up to four (several chunks
+    per thread and both of its buffers at 2^21 and above, with conversion
     between complex64 files and complex128 fields and back): the detector rows
     match the oracle, which loads the whole file; a truncated file is an
     error before anything runs."""
