@@ -361,7 +361,14 @@ def cpu_baseline(N, budget_s=25.0):
         if el > budget_s or n >= 8:
             break
     per_step = el / n
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": 3600.0 / (194 * per_step), "unit": "sims/hour", "cores": 1, "kind": "port",
+            "cpu_model": cpu, "nproc": os.cpu_count(),
             "sample": f"{n} grating slice(s) (mask + FFT, propagator, cutoff, iFFT) of G0 at N=2^{int(math.log2(N))} "
                       f"complex128, scipy.fft 1 worker, {per_step:.2f} s/step, extrapolated x194 steps per sim"}
 
