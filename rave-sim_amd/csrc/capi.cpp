@@ -163,9 +163,14 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
     else if (o.dtype != 0) throw rs::Error(RS_ERR_INVALID, "dtype must be 0, RS_C8 or RS_C16");
     const std::string sub = std::string(sim_dir) + "/" + rs::subdir_name(source_idx);
     rs::Member m;
-    std::vector<std::vector<char>> saved;
     const bool save_u = c.save_final_u_vectors && o.write_output;
-    if (save_u) m.saved_u = &saved;
+    // u vectors and keypoints stream from the device into their files
+    if (save_u)
+      m.saveu_file = [&](int s) {
+        char name[32];
+        std::snprintf(name, sizeof name, "/u_%04d.npy", s);
+        return sub + name;
+      };
     if (c.source.kind == rs::SourceCfg::Vector) {
       // checked here, streamed to the device in the run (Context::stream_npy)
       const rs::NpyHeader h = rs::npy_header(c.source.input_path);
@@ -183,8 +188,12 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
     rs::Recording rec;
     rec.history_dz = o.history_dz;
     rec.keypoints = o.keypoints_dir != nullptr;
-    std::vector<std::vector<char>> keypoints;
-    if (rec.keypoints) m.keypoints = &keypoints;
+    if (rec.keypoints)
+      m.keypoint_file = [&](int k) {
+        char name[48];
+        std::snprintf(name, sizeof name, "/keypoint_%02d_%d.npy", k / 2, k % 2);
+        return std::string(o.keypoints_dir) + name;
+      };
     const int nsteps = rs::build_member(c, m, 0, save_u, &rec);
     rs::RunConfig rc;
     rc.N = c.N;
@@ -217,11 +226,6 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
     }
     if (o.write_output) {
       rs::npy_save(sub + "/detected.npy", "<f8", {nsteps, npx}, det.data(), ndet * sizeof(double));
-      for (size_t s = 0; s < saved.size(); ++s) {
-        char name[32];
-        std::snprintf(name, sizeof name, "/u_%04d.npy", (int)s);
-        rs::npy_save(sub + name, c.f64 ? "<c16" : "<c8", {c.N}, saved[s].data(), saved[s].size());
-      }
     }
     if (o.write_output && rec.history_dz > 0) {
       // np.stack(entries, axis=-1): (n_px, entries); np.empty((0, 0)) when none
@@ -237,13 +241,6 @@ int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run
       rs::npy_save(sub + "/history_z.npy", "<f8", {H}, rec.hist_z.data(), rec.hist_z.size() * sizeof(double));
       const std::vector<double> hx = rs::detector_x(c.detector_size, c.px);
       rs::npy_save(sub + "/history_x.npy", "<f8", {(int64_t)hx.size()}, hx.data(), hx.size() * sizeof(double));
-    }
-    for (size_t k = 0; k < keypoints.size(); ++k) {
-      if (keypoints[k].empty()) continue;
-      char name[48];
-      std::snprintf(name, sizeof name, "/keypoint_%02d_%d.npy", (int)(k / 2), (int)(k % 2));
-      rs::npy_save(std::string(o.keypoints_dir) + name, c.f64 ? "<c16" : "<c8", {c.N}, keypoints[k].data(),
-                   keypoints[k].size());
     }
   });
 }

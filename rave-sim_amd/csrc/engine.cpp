@@ -16,6 +16,7 @@
 #include <sstream>
 
 #include "exec.hpp"
+#include "npy.hpp"
 #include "rs_kernels.hpp"
 
 #pragma STDC FP_CONTRACT OFF
@@ -387,14 +388,7 @@ void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStr
   } closer{fd};
   const size_t desz = f64 ? 16 : 8, sesz = m.upload_c16 ? 16 : 8;
   const int64_t per = (int64_t)(kStageBytes / desz);  // samples per chunk
-  // one thread per 32 MiB of field, at most kStageThreads
-  const int T = (int)std::max<int64_t>(1, std::min<int64_t>(kStageThreads, (N * (int64_t)desz) >> 25));
-  for (int t = 0; t < T; ++t)
-    for (int b = 0; b < 2; ++b) {
-      if (!stage_[t].buf[b])
-        hip_check(hipHostMalloc(&stage_[t].buf[b], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
-      if (!stage_[t].ev[b]) hip_check(hipEventCreateWithFlags(&stage_[t].ev[b], hipEventDisableTiming), "event");
-    }
+  const int T = stage_threads(N * (int64_t)desz);
   std::vector<std::exception_ptr> err(T);
   auto part = [&](int t) {
     try {
@@ -430,6 +424,82 @@ void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStr
         hip_check(hipEventRecord(sg.ev[b], st), "event record");
         sg.used[b] = true;
       }
+    } catch (...) {
+      err[t] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < T; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto &x : th) x.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
+}
+
+int Context::stage_threads(int64_t bytes) {
+  // one thread per 32 MiB of field, at most kStageThreads
+  const int T = (int)std::max<int64_t>(1, std::min<int64_t>(kStageThreads, bytes >> 25));
+  for (int t = 0; t < T; ++t)
+    for (int b = 0; b < 2; ++b) {
+      if (!stage_[t].buf[b])
+        hip_check(hipHostMalloc(&stage_[t].buf[b], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
+      if (!stage_[t].ev[b]) hip_check(hipEventCreateWithFlags(&stage_[t].ev[b], hipEventDisableTiming), "event");
+    }
+  return T;
+}
+
+namespace {
+void pwrite_full(int fd, const char *p, size_t n, int64_t at, const std::string &path) {
+  while (n > 0) {
+    const ssize_t w = ::pwrite(fd, p, n, (off_t)at);
+    if (w <= 0) throw Error(3, "write failed: " + path);
+    p += w;
+    n -= (size_t)w;
+    at += w;
+  }
+}
+}  // namespace
+
+void Context::stream_to_npy(const void *src, int64_t N, bool f64, const std::string &path, hipStream_t st) {
+  const std::string hdr = npy_header_bytes(f64 ? "<c16" : "<c8", {N});
+  const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (fd < 0) throw Error(3, "cannot write " + path);
+  struct Closer {
+    int fd;
+    ~Closer() { ::close(fd); }
+  } closer{fd};
+  pwrite_full(fd, hdr.data(), hdr.size(), 0, path);
+  const size_t esz = f64 ? 16 : 8;
+  const int64_t per = (int64_t)(kStageBytes / esz);
+  const int T = stage_threads(N * (int64_t)esz);
+  std::vector<std::exception_ptr> err(T);
+  auto part = [&](int t) {
+    try {
+      hip_check(hipSetDevice(device), "hipSetDevice");
+      Stage &sg = stage_[t];
+      const int64_t lo = N * t / T, hi = N * (t + 1) / T;
+      int64_t prev_off = -1, prev_n = 0;
+      int prev_b = 0, k = 0;
+      auto flush = [&]() {
+        hip_check(hipEventSynchronize(sg.ev[prev_b]), "staging wait");
+        pwrite_full(fd, (const char *)sg.buf[prev_b], (size_t)prev_n * esz, (int64_t)hdr.size() + prev_off * (int64_t)esz,
+                    path);
+      };
+      for (int64_t off = lo; off < hi; off += per, ++k) {
+        const int b = k & 1;
+        const int64_t n = std::min(per, hi - off);
+        if (sg.used[b]) hip_check(hipEventSynchronize(sg.ev[b]), "staging wait");
+        hip_check(hipMemcpyAsync(sg.buf[b], (const char *)src + (size_t)off * esz, (size_t)n * esz,
+                                 hipMemcpyDeviceToHost, st),
+                  "download");
+        hip_check(hipEventRecord(sg.ev[b], st), "event record");
+        sg.used[b] = true;
+        if (prev_off >= 0) flush();
+        prev_off = off;
+        prev_n = n;
+        prev_b = b;
+      }
+      if (prev_off >= 0) flush();
     } catch (...) {
       err[t] = std::current_exception();
     }

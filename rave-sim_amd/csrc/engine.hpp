@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -133,6 +134,9 @@ struct Member {
   bool upload_c16 = false;      // file elements are complex128 (else complex64)
   std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
   std::vector<std::vector<char>> *keypoints = nullptr;  // Keypoint sink (indexed by the op's row)
+  // instead of the sinks: .npy files the SaveU / Keypoint ops stream the
+  // field into (path of a step / row), through the context's staging buffers
+  std::function<std::string(int)> saveu_file, keypoint_file;
 };
 
 struct RunConfig {
@@ -203,6 +207,12 @@ class Context {
   // pinned staging buffers in turn, each reused once its previous copy has
   // left it, so file reads overlap the transfers and each other.
   void stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st);
+  // The reverse: N natural-order samples of the device field src, after the
+  // work already on stream st, into a new .npy file at path (complex64 or
+  // complex128 as f64), each thread downloading chunks of its part into its
+  // two staging buffers in turn and writing one while the next arrives.
+  // Returns when the file is complete.
+  void stream_to_npy(const void *src, int64_t N, bool f64, const std::string &path, hipStream_t st);
 
  private:
   static constexpr size_t kStageBytes = size_t(4) << 20;
@@ -213,6 +223,8 @@ class Context {
     bool used[2] = {false, false};
   };
   Stage stage_[kStageThreads];
+  // staging threads for a field of `bytes`, with their buffers allocated
+  int stage_threads(int64_t bytes);
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;

@@ -582,6 +582,11 @@ template <typename S> struct Exec {
         case LKind::SaveU: {
           end_capture();
           Member *m = mem[l.member];
+          const auto &file = l.keypoint ? m->keypoint_file : m->saveu_file;
+          if (file) {
+            ctx.stream_to_npy(l.src, l.n, sizeof(S) == 8, file(l.step), st);
+            break;
+          }
           std::vector<std::vector<char>> *sink = l.keypoint ? m->keypoints : m->saved_u;
           if (sink) {
             std::vector<char> host(l.n * esz);

@@ -98,8 +98,7 @@ NpyArray npy_load(const std::string &path) {
   return a;
 }
 
-void npy_save(const std::string &path, const std::string &descr, const std::vector<int64_t> &shape,
-              const void *data, size_t bytes) {
+std::string npy_header_bytes(const std::string &descr, const std::vector<int64_t> &shape) {
   std::ostringstream h;
   h << "{'descr': '" << descr << "', 'fortran_order': False, 'shape': (";
   for (size_t i = 0; i < shape.size(); ++i) {
@@ -113,12 +112,16 @@ void npy_save(const std::string &path, const std::string &descr, const std::vect
   const size_t pad = (64 - total % 64) % 64;
   hs += std::string(pad, ' ');
   hs += '\n';
+  const char pre[10] = {'\x93', 'N', 'U', 'M', 'P', 'Y', 1, 0, (char)(hs.size() & 0xff), (char)(hs.size() >> 8)};
+  return std::string(pre, 10) + hs;
+}
+
+void npy_save(const std::string &path, const std::string &descr, const std::vector<int64_t> &shape,
+              const void *data, size_t bytes) {
+  const std::string hdr = npy_header_bytes(descr, shape);
   std::ofstream f(path, std::ios::binary);
   if (!f) throw Error(3, "cannot write " + path);
-  const unsigned char hdr[10] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0, (unsigned char)(hs.size() & 0xff),
-                                 (unsigned char)(hs.size() >> 8)};
-  f.write((const char *)hdr, 10);
-  f.write(hs.data(), (std::streamsize)hs.size());
+  f.write(hdr.data(), (std::streamsize)hdr.size());
   f.write((const char *)data, (std::streamsize)bytes);
   if (!f) throw Error(3, "write failed: " + path);
 }

@@ -28,6 +28,9 @@ struct NpyHeader {
 };
 
 NpyHeader npy_header(const std::string &path);
+// the bytes of a version 1.0 .npy header for a C-order array (the data
+// follows them directly)
+std::string npy_header_bytes(const std::string &descr, const std::vector<int64_t> &shape);
 NpyArray npy_load(const std::string &path);
 void npy_save(const std::string &path, const std::string &descr, const std::vector<int64_t> &shape,
               const void *data, size_t bytes);
