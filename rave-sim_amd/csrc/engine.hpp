@@ -132,10 +132,9 @@ struct Member {
   std::string upload_npy;       // path
   int64_t upload_offset = 0;    // byte offset of the data in the file
   bool upload_c16 = false;      // file elements are complex128 (else complex64)
-  std::vector<std::vector<char>> *saved_u = nullptr;  // SaveU sink (one entry per step)
-  std::vector<std::vector<char>> *keypoints = nullptr;  // Keypoint sink (indexed by the op's row)
-  // instead of the sinks: .npy files the SaveU / Keypoint ops stream the
-  // field into (path of a step / row), through the context's staging buffers
+  // .npy files the SaveU / Keypoint ops stream the field into (path of a
+  // phase step / keypoint row), through the context's staging buffers; the
+  // ops do nothing when unset
   std::function<std::string(int)> saveu_file, keypoint_file;
 };
 

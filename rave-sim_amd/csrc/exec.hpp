@@ -30,7 +30,7 @@ struct Launch {
   int64_t n = 0;
   int member = 0;
   int step = 0;
-  bool keypoint = false;  // SaveU: into the member's keypoint sink
+  bool keypoint = false;  // SaveU: a keypoint (the member's keypoint_file), not a u vector
 };
 
 // pending state of one lockstep group
@@ -583,18 +583,7 @@ template <typename S> struct Exec {
           end_capture();
           Member *m = mem[l.member];
           const auto &file = l.keypoint ? m->keypoint_file : m->saveu_file;
-          if (file) {
-            ctx.stream_to_npy(l.src, l.n, sizeof(S) == 8, file(l.step), st);
-            break;
-          }
-          std::vector<std::vector<char>> *sink = l.keypoint ? m->keypoints : m->saved_u;
-          if (sink) {
-            std::vector<char> host(l.n * esz);
-            hip_check(hipMemcpyAsync(host.data(), l.src, l.n * esz, hipMemcpyDeviceToHost, st), "download u");
-            hip_check(hipStreamSynchronize(st), "sync");
-            if ((int)sink->size() <= l.step) sink->resize(l.step + 1);
-            (*sink)[l.step] = std::move(host);
-          }
+          if (file) ctx.stream_to_npy(l.src, l.n, sizeof(S) == 8, file(l.step), st);
           break;
         }
       }
