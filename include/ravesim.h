@@ -46,7 +46,10 @@ extern "C" {
 typedef struct rs_ctx rs_ctx;
 typedef struct rs_field rs_field;
 
-/* Library version (major*10000 + minor*100 + patch). */
+/* Library version (major*10000 + minor*100 + patch).  The minor number
+ * changes with the layout of any struct below; a binding checks it before
+ * passing structs. */
+#define RS_VERSION 200
 int rs_version(void);
 
 /* Message of this thread's last failed call ("" if none). */

@@ -125,7 +125,7 @@ rs::GratingCfg dba_grating_from(const rs_dba_grating *g) {
 
 extern "C" {
 
-int rs_version(void) { return 100; }
+int rs_version(void) { return RS_VERSION; }
 
 const char *rs_last_error(void) { return g_err.c_str(); }
 

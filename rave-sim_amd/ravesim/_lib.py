@@ -113,6 +113,10 @@ class rs_pass_stat(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
+# include/ravesim.h RS_VERSION: the struct layouts declared above
+VERSION = 200
+
+
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run `make -C {HERE.parent}` (hipcc, gfx950)")
@@ -160,6 +164,9 @@ def _load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.rs_version() != VERSION:
+        raise ImportError(f"{LIB_PATH} is version {lib.rs_version()}, this binding's structs are version "
+                          f"{VERSION}; rebuild with `make -C {HERE.parent}`")
     return lib
 
 

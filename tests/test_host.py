@@ -23,7 +23,8 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(_lib.EXPORTS)
     for name in declared:
         assert hasattr(_lib.lib, name), name
-    assert _lib.lib.rs_version() >= 100
+    assert _lib.lib.rs_version() == _lib.VERSION
+    assert re.search(r"#define RS_VERSION (\d+)", header).group(1) == str(_lib.VERSION)
 
 
 def test_ctypes_structs_match_the_header(tmp_path):
