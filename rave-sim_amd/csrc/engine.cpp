@@ -204,6 +204,11 @@ void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, 
     rest = lgN - r;
   }
   int a = lgA_req >= 0 ? lgA_req : (rest + 1) / 2;
+  // untuned complex64 plans keep the top column at 256 or less, where its
+  // tiles read 256 B row segments (2^28: 635 ms per Talbot-Lau simulation
+  // with 12|8|8 against 689 with 12|9|7; 2^30: 3.1 s with 12|8|10 against
+  // 3.3 s with 12|9|9)
+  if (lgA_req < 0 && !f64) a = std::min(a, 8);
   int b = rest - a;
   if (a > maxW || a < kMinColLg || b < kMinColLg || b > maxW)
     throw Error(6, "no three-group split for this N");
