@@ -303,9 +303,9 @@ int rs_field_upload(rs_field *f, const void *host) {
 int rs_field_download(rs_field *f, void *host) {
   if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
   return guarded([&] {
-    rs::hip_check(hipMemcpyAsync(host, f->buf.p, f->buf.bytes, hipMemcpyDeviceToHost, f->owner->ctx->stream),
-                  "download");
-    rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
+    // through the context's pinned staging buffers: pageable destinations
+    // get the transfer rate of pinned ones
+    f->owner->ctx->copy_to_host(host, f->buf.p, f->buf.bytes, f->owner->ctx->stream);
   });
 }
 

@@ -465,46 +465,36 @@ void pwrite_full(int fd, const char *p, size_t n, int64_t at, const std::string 
 }
 }  // namespace
 
-void Context::stream_to_npy(const void *src, int64_t N, bool f64, const std::string &path, hipStream_t st) {
-  const std::string hdr = npy_header_bytes(f64 ? "<c16" : "<c8", {N});
-  const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
-  if (fd < 0) throw Error(3, "cannot write " + path);
-  struct Closer {
-    int fd;
-    ~Closer() { ::close(fd); }
-  } closer{fd};
-  pwrite_full(fd, hdr.data(), hdr.size(), 0, path);
-  const size_t esz = f64 ? 16 : 8;
-  const int64_t per = (int64_t)(kStageBytes / esz);
-  const int T = stage_threads(N * (int64_t)esz);
+void Context::stream_out(const void *src, size_t bytes, hipStream_t st,
+                         const std::function<void(const char *, size_t, size_t)> &sink) {
+  const int T = stage_threads((int64_t)bytes);
   std::vector<std::exception_ptr> err(T);
   auto part = [&](int t) {
     try {
       hip_check(hipSetDevice(device), "hipSetDevice");
       Stage &sg = stage_[t];
-      const int64_t lo = N * t / T, hi = N * (t + 1) / T;
-      int64_t prev_off = -1, prev_n = 0;
+      const size_t lo = bytes * t / T, hi = bytes * (t + 1) / T;
+      size_t prev_off = 0, prev_n = 0;
       int prev_b = 0, k = 0;
+      bool have = false;
       auto flush = [&]() {
         hip_check(hipEventSynchronize(sg.ev[prev_b]), "staging wait");
-        pwrite_full(fd, (const char *)sg.buf[prev_b], (size_t)prev_n * esz, (int64_t)hdr.size() + prev_off * (int64_t)esz,
-                    path);
+        sink((const char *)sg.buf[prev_b], prev_off, prev_n);
       };
-      for (int64_t off = lo; off < hi; off += per, ++k) {
+      for (size_t off = lo; off < hi; off += kStageBytes, ++k) {
         const int b = k & 1;
-        const int64_t n = std::min(per, hi - off);
+        const size_t n = std::min(kStageBytes, hi - off);
         if (sg.used[b]) hip_check(hipEventSynchronize(sg.ev[b]), "staging wait");
-        hip_check(hipMemcpyAsync(sg.buf[b], (const char *)src + (size_t)off * esz, (size_t)n * esz,
-                                 hipMemcpyDeviceToHost, st),
-                  "download");
+        hip_check(hipMemcpyAsync(sg.buf[b], (const char *)src + off, n, hipMemcpyDeviceToHost, st), "download");
         hip_check(hipEventRecord(sg.ev[b], st), "event record");
         sg.used[b] = true;
-        if (prev_off >= 0) flush();
+        if (have) flush();
         prev_off = off;
         prev_n = n;
         prev_b = b;
+        have = true;
       }
-      if (prev_off >= 0) flush();
+      if (have) flush();
     } catch (...) {
       err[t] = std::current_exception();
     }
@@ -515,6 +505,24 @@ void Context::stream_to_npy(const void *src, int64_t N, bool f64, const std::str
   for (auto &x : th) x.join();
   for (auto &e : err)
     if (e) std::rethrow_exception(e);
+}
+
+void Context::stream_to_npy(const void *src, int64_t N, bool f64, const std::string &path, hipStream_t st) {
+  const std::string hdr = npy_header_bytes(f64 ? "<c16" : "<c8", {N});
+  const int fd = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+  if (fd < 0) throw Error(3, "cannot write " + path);
+  struct Closer {
+    int fd;
+    ~Closer() { ::close(fd); }
+  } closer{fd};
+  pwrite_full(fd, hdr.data(), hdr.size(), 0, path);
+  stream_out(src, (size_t)N * (f64 ? 16 : 8), st, [&](const char *p, size_t off, size_t n) {
+    pwrite_full(fd, p, n, (int64_t)(hdr.size() + off), path);
+  });
+}
+
+void Context::copy_to_host(void *host, const void *src, size_t bytes, hipStream_t st) {
+  stream_out(src, bytes, st, [&](const char *p, size_t off, size_t n) { std::memcpy((char *)host + off, p, n); });
 }
 
 hipStream_t Context::lane_stream(int i) {

@@ -212,6 +212,10 @@ class Context {
   // two staging buffers in turn and writing one while the next arrives.
   // Returns when the file is complete.
   void stream_to_npy(const void *src, int64_t N, bool f64, const std::string &path, hipStream_t st);
+  // Device-to-host copy of `bytes` after the work already on stream st,
+  // through the staging buffers (pageable destinations included); returns
+  // when host holds the data.
+  void copy_to_host(void *host, const void *src, size_t bytes, hipStream_t st);
 
  private:
   static constexpr size_t kStageBytes = size_t(4) << 20;
@@ -224,6 +228,12 @@ class Context {
   Stage stage_[kStageThreads];
   // staging threads for a field of `bytes`, with their buffers allocated
   int stage_threads(int64_t bytes);
+  // the download loop of stream_to_npy / copy_to_host: each thread takes a
+  // contiguous part of the `bytes` at src, downloads it chunk by chunk into
+  // its two staging buffers in turn and hands each chunk to sink(data,
+  // byte offset, length) while the next one arrives
+  void stream_out(const void *src, size_t bytes, hipStream_t st,
+                  const std::function<void(const char *, size_t, size_t)> &sink);
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;
