@@ -294,8 +294,7 @@ int rs_field_destroy(rs_field *f) {
 int rs_field_upload(rs_field *f, const void *host) {
   if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
   return guarded([&] {
-    rs::hip_check(hipMemcpyAsync(f->buf.p, host, f->buf.bytes, hipMemcpyHostToDevice, f->owner->ctx->stream),
-                  "upload");
+    f->owner->ctx->copy_from_host(f->buf.p, host, f->buf.bytes, f->owner->ctx->stream);
     rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
   });
 }

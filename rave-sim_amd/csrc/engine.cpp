@@ -384,47 +384,24 @@ Context::~Context() {
   for (hipStream_t s : lanes_) (void)hipStreamDestroy(s);
   if (stream) (void)hipStreamDestroy(stream);
 }
-void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st) {
-  const int fd = ::open(m.upload_npy.c_str(), O_RDONLY);
-  if (fd < 0) throw Error(3, "cannot open " + m.upload_npy);
-  struct Closer {
-    int fd;
-    ~Closer() { ::close(fd); }
-  } closer{fd};
-  const size_t desz = f64 ? 16 : 8, sesz = m.upload_c16 ? 16 : 8;
-  const int64_t per = (int64_t)(kStageBytes / desz);  // samples per chunk
-  const int T = stage_threads(N * (int64_t)desz);
+void Context::stream_in(void *dst, int64_t count, size_t esz, hipStream_t st,
+                        const std::function<void(char *, int64_t, int64_t)> &fill) {
+  const int64_t per = (int64_t)(kStageBytes / esz);  // elements per chunk
+  const int T = stage_threads(count * (int64_t)esz);
   std::vector<std::exception_ptr> err(T);
   auto part = [&](int t) {
     try {
       hip_check(hipSetDevice(device), "hipSetDevice");
       Stage &sg = stage_[t];
-      std::vector<char> conv(sesz != desz ? (size_t)per * sesz : 0);
-      const int64_t lo = N * t / T, hi = N * (t + 1) / T;
+      const int64_t lo = count * t / T, hi = count * (t + 1) / T;
       int k = 0;
       for (int64_t off = lo; off < hi; off += per, ++k) {
         const int b = k & 1;
         const int64_t n = std::min(per, hi - off);
         if (sg.used[b]) hip_check(hipEventSynchronize(sg.ev[b]), "staging wait");
-        char *buf = (char *)sg.buf[b];
-        char *rd = sesz == desz ? buf : conv.data();
-        size_t got = 0;
-        const size_t want = (size_t)n * sesz;
-        while (got < want) {
-          const ssize_t r = ::pread(fd, rd + got, want - got, (off_t)(m.upload_offset + off * (int64_t)sesz + got));
-          if (r <= 0) throw Error(3, m.upload_npy + ": truncated data");
-          got += (size_t)r;
-        }
-        if (sesz == 16 && desz == 8) {
-          const double *s = (const double *)rd;
-          float *d = (float *)buf;
-          for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
-        } else if (sesz == 8 && desz == 16) {
-          const float *s = (const float *)rd;
-          double *d = (double *)buf;
-          for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
-        }
-        hip_check(hipMemcpyAsync((char *)dst + (size_t)off * desz, buf, (size_t)n * desz, hipMemcpyHostToDevice, st),
+        fill((char *)sg.buf[b], off, n);
+        hip_check(hipMemcpyAsync((char *)dst + (size_t)off * esz, sg.buf[b], (size_t)n * esz, hipMemcpyHostToDevice,
+                                 st),
                   "upload");
         hip_check(hipEventRecord(sg.ev[b], st), "event record");
         sg.used[b] = true;
@@ -439,6 +416,49 @@ void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStr
   for (auto &x : th) x.join();
   for (auto &e : err)
     if (e) std::rethrow_exception(e);
+}
+
+void Context::stream_npy(const Member &m, bool f64, int64_t N, void *dst, hipStream_t st) {
+  const int fd = ::open(m.upload_npy.c_str(), O_RDONLY);
+  if (fd < 0) throw Error(3, "cannot open " + m.upload_npy);
+  struct Closer {
+    int fd;
+    ~Closer() { ::close(fd); }
+  } closer{fd};
+  const size_t desz = f64 ? 16 : 8, sesz = m.upload_c16 ? 16 : 8;
+  auto read_at = [&](char *p, size_t want, int64_t at) {
+    size_t got = 0;
+    while (got < want) {
+      const ssize_t r = ::pread(fd, p + got, want - got, (off_t)(at + (int64_t)got));
+      if (r <= 0) throw Error(3, m.upload_npy + ": truncated data");
+      got += (size_t)r;
+    }
+  };
+  stream_in(dst, N, desz, st, [&](char *buf, int64_t first, int64_t n) {
+    const int64_t at = m.upload_offset + first * (int64_t)sesz;
+    if (sesz == desz) {
+      read_at(buf, (size_t)n * sesz, at);
+      return;
+    }
+    // precision conversion through a per-thread scratch chunk
+    thread_local std::vector<char> raw;
+    raw.resize((size_t)n * sesz);
+    read_at(raw.data(), raw.size(), at);
+    if (sesz == 16) {
+      const double *s = (const double *)raw.data();
+      float *d = (float *)buf;
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (float)s[i];
+    } else {
+      const float *s = (const float *)raw.data();
+      double *d = (double *)buf;
+      for (int64_t i = 0; i < 2 * n; ++i) d[i] = (double)s[i];
+    }
+  });
+}
+
+void Context::copy_from_host(void *dst, const void *host, size_t bytes, hipStream_t st) {
+  stream_in(dst, (int64_t)bytes, 1, st,
+            [&](char *buf, int64_t first, int64_t n) { std::memcpy(buf, (const char *)host + first, (size_t)n); });
 }
 
 int Context::stage_threads(int64_t bytes) {

@@ -126,8 +126,7 @@ struct Member {
   int out_base = 0;             // accumulator row of Detect(row = 0)
   std::vector<Op> ops;
   std::vector<SampleRows> samples;  // device copies are made per run
-  const void *upload = nullptr; // host data for the Upload op (natural order)
-  // or, when upload is null, a .npy vector streamed from disk through the
+  // Upload op: a .npy vector (natural order) streamed from disk through the
   // context's pinned staging buffers (no host copy of the whole field)
   std::string upload_npy;       // path
   int64_t upload_offset = 0;    // byte offset of the data in the file
@@ -216,6 +215,9 @@ class Context {
   // through the staging buffers (pageable destinations included); returns
   // when host holds the data.
   void copy_to_host(void *host, const void *src, size_t bytes, hipStream_t st);
+  // Host-to-device copy of `bytes` through the staging buffers, ordered on
+  // stream st; host may be reused when it returns.
+  void copy_from_host(void *dst, const void *host, size_t bytes, hipStream_t st);
 
  private:
   static constexpr size_t kStageBytes = size_t(4) << 20;
@@ -234,6 +236,12 @@ class Context {
   // byte offset, length) while the next one arrives
   void stream_out(const void *src, size_t bytes, hipStream_t st,
                   const std::function<void(const char *, size_t, size_t)> &sink);
+  // the upload loop of stream_npy / copy_from_host: each thread takes a
+  // contiguous part of the `count` elements of `esz` bytes, has fill(buffer,
+  // first element, count) put each chunk into its staging buffers in turn and
+  // copies it to dst while it fills the next
+  void stream_in(void *dst, int64_t count, size_t esz, hipStream_t st,
+                 const std::function<void(char *, int64_t, int64_t)> &fill);
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
   std::vector<std::pair<int, int>> plan_req_;

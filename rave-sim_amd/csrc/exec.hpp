@@ -569,10 +569,8 @@ template <typename S> struct Exec {
         case LKind::Upload: {
           end_capture();
           const Member &m = *mem[l.member];
-          if (m.upload)
-            hip_check(hipMemcpyAsync(l.dst, m.upload, l.n * esz, hipMemcpyHostToDevice, st), "upload");
-          else
-            ctx.stream_npy(m, sizeof(S) == 8, l.n, l.dst, st);
+          if (m.upload_npy.empty()) throw Error(1, "upload op without a source file");
+          ctx.stream_npy(m, sizeof(S) == 8, l.n, l.dst, st);
           break;
         }
         case LKind::Fill:
