@@ -305,6 +305,16 @@ def test_fastwave_cli_arguments():
     assert r.returncode == 1 and "history_dz" in r.stderr
 
 
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits 2 with a message, before
+    touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
 def test_malformed_yaml_is_a_config_error(tmp_path):
     """A config.yaml that is not YAML is reported, not crashed on."""
     from ravesim import _lib
