@@ -126,6 +126,27 @@ def test_grating_mask_cached_bit_exact(ctx, N, xpos, dtype):
     assert np.all(np.abs(a[exp] - 1.0 / 16.0) < 1e-3) and np.all(np.abs(a[~exp] - 1.0) < 1e-3)
 
 
+@pytest.mark.parametrize("N,dtype", [(64, np.complex64), (2 ** 20, np.complex128), (2 ** 26, np.complex64),
+                                     (2 ** 25, np.complex128)])
+def test_upload_download_round_trip(ctx, N, dtype):
+    """rs_field_upload / rs_field_download go through the context's pinned
+    staging buffers (one to four copying threads by size): the field comes
+    back bit for bit, and a copy on the device matches too."""
+    import ravesim
+    rng = np.random.default_rng(N)
+    u = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(dtype)
+    f = ravesim.Field(ctx, N, 2e-9, dtype=dtype)
+    f.upload(u)
+    assert np.array_equal(f.download(), u)
+    g = ravesim.Field(ctx, N, 2e-9, dtype=dtype)
+    f.copy_to(g)
+    f.upload(np.zeros(N, dtype=dtype))
+    assert np.array_equal(g.download(), u)
+    assert not np.any(f.download())
+    f.close()
+    g.close()
+
+
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_point_source_matches_oracle(ctx, dtype):
     import ravesim
