@@ -15,7 +15,9 @@
  *
  * Threading: one host thread at a time per rs_ctx; distinct contexts may be
  * used from distinct threads.  All work of a context runs on its own HIP
- * stream.
+ * streams.  A server (rs_server_create) reserves its context until
+ * rs_server_destroy: meanwhile every other call that uses the context, its
+ * fields included, returns RS_ERR_INVALID (rs_ctx_synchronize excepted).
  */
 #ifndef RAVESIM_H
 #define RAVESIM_H
@@ -295,9 +297,15 @@ typedef struct rs_request {
   double source_x, source_z;
 } rs_request;
 
-/* Start a scheduler thread on the context's device.  reserve_bytes of free
- * device memory are left unused (<= 0: 4 GiB); max_jobs bounds the number of
- * (request, energy) jobs resident at once (<= 0: no bound). */
+/* Start a scheduler thread on the context's device; the context is reserved
+ * for it until rs_server_destroy.  Jobs ((request, energy) pairs) are
+ * admitted in arrival order while their buffers fit in the free device
+ * memory, read at each admission, less reserve_bytes (<= 0: 4 GiB); a job
+ * that cannot fit with nothing else resident fails its request (poll reports
+ * state 3 and returns RS_ERR_DEVICE with an RS_ERR_NOMEM message).  max_jobs
+ * bounds the number of jobs resident at once (<= 0: no bound).  Grid shapes
+ * are spread over up to four streams, so requests of different N run
+ * concurrently. */
 int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out);
 /* Queue a request (all inputs copied); returns immediately. */
 int rs_server_submit(rs_server *srv, const rs_request *req, int64_t *id);

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -17,10 +18,15 @@
 
 struct rs_ctx {
   std::unique_ptr<rs::Context> ctx;
+  // servers attached to the context: while one lives, its scheduler thread
+  // owns the context's streams and plan cache, and the other entry points
+  // that use the context refuse to run
+  std::atomic<int> servers{0};
 };
 
 struct rs_server {
   std::unique_ptr<rs::Server> srv;
+  rs_ctx *owner = nullptr;
 };
 
 struct rs_field {
@@ -38,6 +44,12 @@ thread_local std::string g_err;
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
+}
+
+// the context is reserved by a live server (rs_server_create .. rs_server_destroy)
+bool reserved(const rs_ctx *c) { return c && c->servers.load() > 0; }
+int fail_reserved() {
+  return fail(RS_ERR_INVALID, "the context is reserved by a server until rs_server_destroy");
 }
 
 template <typename F> int guarded(F &&f) {
@@ -143,6 +155,7 @@ int rs_ctx_create(int device, rs_ctx **out) {
 
 int rs_ctx_destroy(rs_ctx *ctx) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (reserved(ctx)) return fail_reserved();
   return guarded([&] { delete ctx; });
 }
 
@@ -153,6 +166,7 @@ int rs_ctx_synchronize(rs_ctx *ctx) {
 
 int rs_run_simdir(rs_ctx *ctx, const char *sim_dir, int source_idx, const rs_run_options *opts) {
   if (!ctx || !sim_dir) return fail(RS_ERR_INVALID, "ctx or sim_dir is NULL");
+  if (reserved(ctx)) return fail_reserved();
   rs_run_options o{};
   o.write_output = 1;
   if (opts) o = *opts;
@@ -265,6 +279,7 @@ int rs_validate_simdir(const char *sim_dir, int source_idx, int64_t *info, int64
 
 int rs_field_create(rs_ctx *ctx, const rs_grid *grid, int dtype, int mode, rs_field **out) {
   if (!ctx || !grid || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
   if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
   if (mode != RS_MODE_FUSED && mode != RS_MODE_PLAIN) return fail(RS_ERR_INVALID, "unknown mode");
   if (grid->N < 64 || (grid->N & (grid->N - 1)) != 0 || grid->N > (int64_t(1) << 30))
@@ -293,6 +308,7 @@ int rs_field_destroy(rs_field *f) {
 
 int rs_field_upload(rs_field *f, const void *host) {
   if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     f->owner->ctx->copy_from_host(f->buf.p, host, f->buf.bytes, f->owner->ctx->stream);
     rs::hip_check(hipStreamSynchronize(f->owner->ctx->stream), "sync");
@@ -301,6 +317,7 @@ int rs_field_upload(rs_field *f, const void *host) {
 
 int rs_field_download(rs_field *f, void *host) {
   if (!f || !host) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     // through the context's pinned staging buffers: pageable destinations
     // get the transfer rate of pinned ones
@@ -312,6 +329,7 @@ int rs_field_copy(rs_field *dst, const rs_field *src) {
   if (!dst || !src) return fail(RS_ERR_INVALID, "NULL argument");
   if (dst->grid.N != src->grid.N || dst->f64 != src->f64 || dst->owner != src->owner)
     return fail(RS_ERR_INVALID, "fields differ in size, precision or context");
+  if (reserved(dst->owner)) return fail_reserved();
   return guarded([&] {
     hipStream_t st = dst->owner->ctx->stream;
     rs::hip_check(dst->f64 ? rs::launch_copy<double>(dst->buf.p, src->buf.p, dst->grid.N, st)
@@ -324,6 +342,7 @@ int rs_field_copy(rs_field *dst, const rs_field *src) {
 int rs_point_source(rs_field *f, double wl, double x_source, double z) {
   if (!f) return fail(RS_ERR_INVALID, "field is NULL");
   if (!(wl > 0)) return fail(RS_ERR_INVALID, "wavelength must be positive");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::Member m;
     m.wl = wl;
@@ -343,6 +362,7 @@ int rs_point_source(rs_field *f, double wl, double x_source, double z) {
 int rs_propagate(rs_field *f, double wl, double dz, double cutoff_freq) {
   if (!f) return fail(RS_ERR_INVALID, "field is NULL");
   if (!(wl > 0)) return fail(RS_ERR_INVALID, "wavelength must be positive");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::Member m;
     m.wl = wl;
@@ -360,6 +380,7 @@ int rs_apply_grating(rs_field *f, double wl, const rs_grating *g, int phase_step
   if (g->nr_steps < 1 || g->nr_x_positions < 1 || !g->x_positions)
     return fail(RS_ERR_INVALID, "grating needs nr_steps >= 1 and at least one x position");
   if (phase_step < 0 || phase_step >= g->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::SimConfig c = field_cfg(f, wl);
     const rs::GratingCfg gc = grating_from(g);
@@ -376,6 +397,7 @@ int rs_apply_dba_grating(rs_field *f, double wl, const rs_dba_grating *g, int ph
   if (g->nr_steps < 1 || g->nr_x_positions < 1 || !g->x_positions)
     return fail(RS_ERR_INVALID, "grating needs nr_steps >= 1 and at least one x position");
   if (phase_step < 0 || phase_step >= g->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::SimConfig c = field_cfg(f, wl);
     const rs::GratingCfg gc = dba_grating_from(g);
@@ -395,6 +417,7 @@ int rs_apply_sample(rs_field *f, double wl, const rs_sample *s, int phase_step, 
     return fail(RS_ERR_INVALID, "sample pixel sizes must be positive");
   if (s->nr_materials < 0 || s->nr_x_positions < 1) return fail(RS_ERR_INVALID, "bad material or position count");
   if (phase_step < 0 || phase_step >= s->nr_x_positions) return fail(RS_ERR_INVALID, "phase_step out of range");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::SimConfig c = field_cfg(f, wl);
     rs::GratingCfg g;
@@ -420,6 +443,7 @@ int rs_apply_sample(rs_field *f, double wl, const rs_sample *s, int phase_step, 
 
 int rs_scale(rs_field *f, double re, double im) {
   if (!f) return fail(RS_ERR_INVALID, "field is NULL");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     rs::Member m;
     m.wl = 1.0;
@@ -441,6 +465,7 @@ int64_t rs_detector_pixels(const rs_detector *det) {
 
 int rs_square_and_downsample(rs_field *f, const rs_detector *det, double z, double *out, int64_t n_out) {
   if (!f || !det || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(f->owner)) return fail_reserved();
   return guarded([&] {
     const int64_t npx = rs::detector_pixels(det->size, det->pixel_x);
     if (npx > f->grid.N) throw rs::Error(RS_ERR_INVALID, "detector has more pixels than samples");
@@ -470,6 +495,7 @@ int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bo
 
 int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (reserved(ctx)) return fail_reserved();
   if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
   if (N < 64 || (N & (N - 1)) != 0 || N > (int64_t(1) << 30))
     return fail(RS_ERR_UNSUPPORTED, "N must be a power of two in [2^6, 2^30]");
@@ -488,6 +514,7 @@ int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, d
 
 int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (reserved(ctx)) return fail_reserved();
   return guarded([&] {
     int lgN = 0;
     while ((int64_t(1) << lgN) < N) ++lgN;
@@ -503,9 +530,17 @@ int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
 
 int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out) {
   if (!ctx || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
   return guarded([&] {
     auto s = std::make_unique<rs_server>();
-    s->srv = std::make_unique<rs::Server>(*ctx->ctx, reserve_bytes, max_jobs);
+    s->owner = ctx;
+    ctx->servers++;
+    try {
+      s->srv = std::make_unique<rs::Server>(*ctx->ctx, reserve_bytes, max_jobs);
+    } catch (...) {
+      ctx->servers--;
+      throw;
+    }
     *out = s.release();
   });
 }
@@ -589,12 +624,15 @@ int rs_server_destroy(rs_server *srv) {
   if (!srv) return fail(RS_ERR_INVALID, "server is NULL");
   return guarded([&] {
     srv->srv->drain();
+    rs_ctx *owner = srv->owner;
     delete srv;
+    owner->servers--;
   });
 }
 
 int rs_set_timing(rs_ctx *ctx, int enable) {
   if (!ctx) return fail(RS_ERR_INVALID, "ctx is NULL");
+  if (reserved(ctx)) return fail_reserved();
   if (enable < 0) return fail(RS_ERR_INVALID, "timing stride must be >= 0");
   ctx->ctx->timing = enable != 0;
   ctx->ctx->timing_stride = enable > 0 ? enable : 1;
@@ -615,6 +653,7 @@ int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n) {
 
 int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out) {
   if (!ctx || !d || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
   if (d->nr_members < 1 || !d->members) return fail(RS_ERR_INVALID, "batch has no members");
   if (d->dtype != RS_C8 && d->dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
   return guarded([&] {

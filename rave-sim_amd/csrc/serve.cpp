@@ -18,18 +18,27 @@ double now_s() {
 
 // fields per launch assumed when picking a tuned split for served jobs
 constexpr int kServeBatchHint = 16;
+// lanes (streams) the grid shapes are spread over, argument slots per lane
+constexpr int kLanes = 4;
+constexpr int kSlotsPerLane = 3;
+// a job of N samples advances by kPaceSamples / N passes per tick (at least
+// one, at most kMaxPassesPerTick): about the same device time per tick
+constexpr int64_t kPaceSamples = int64_t(1) << 26;
+constexpr int kMaxPassesPerTick = 64;
 
 }  // namespace
 
 // device / pinned buffers reused across jobs (hipFree would stall the stream)
 struct Server::Pool {
   std::map<size_t, std::vector<void *>> dev, host;
-  size_t dev_bytes = 0;
+  size_t dev_bytes = 0;   // allocated by the pool, in use or idle
+  size_t idle_bytes = 0;  // cached in the free lists
   void *get_dev(size_t b) {
     auto &v = dev[b];
     if (!v.empty()) {
       void *p = v.back();
       v.pop_back();
+      idle_bytes -= b;
       return p;
     }
     void *p = nullptr;
@@ -37,7 +46,10 @@ struct Server::Pool {
     dev_bytes += b;
     return p;
   }
-  void put_dev(size_t b, void *p) { dev[b].push_back(p); }
+  void put_dev(size_t b, void *p) {
+    dev[b].push_back(p);
+    idle_bytes += b;
+  }
   void *get_host(size_t b) {
     auto &v = host[b];
     if (!v.empty()) {
@@ -50,13 +62,29 @@ struct Server::Pool {
     return p;
   }
   void put_host(size_t b, void *p) { host[b].push_back(p); }
-  // release cached device buffers of sizes other than `keep` until `need` fits
-  void trim(size_t budget, size_t need) {
+  // bytes of the buffers `sizes` that idle buffers of exactly those sizes
+  // would supply (no new allocation)
+  size_t reusable(const std::vector<size_t> &sizes) const {
+    std::map<size_t, size_t> want;
+    for (size_t b : sizes) want[b]++;
+    size_t r = 0;
+    for (auto &kv : want) {
+      auto it = dev.find(kv.first);
+      const size_t have = it == dev.end() ? 0 : it->second.size();
+      r += kv.first * std::min(have, kv.second);
+    }
+    return r;
+  }
+  // release idle cached device buffers, other than one of each of `keep`'s
+  // sizes per entry, until at most `want` bytes are idle
+  void trim(size_t want, const std::vector<size_t> &keep) {
     for (auto &kv : dev) {
-      while (!kv.second.empty() && dev_bytes + need > budget) {
+      const size_t k = (size_t)std::count(keep.begin(), keep.end(), kv.first);
+      while (kv.second.size() > k && idle_bytes > want) {
         (void)hipFree(kv.second.back());
         kv.second.pop_back();
         dev_bytes -= kv.first;
+        idle_bytes -= kv.first;
       }
     }
   }
@@ -84,6 +112,8 @@ struct Server::Job {
   size_t member = 0;
   bool f64 = false;
   int64_t N = 0;
+  int lane = 0;
+  int pace = 1;  // passes per tick
   Member m;
   void *field = nullptr, *snap = nullptr;
   double *acc = nullptr;
@@ -105,20 +135,43 @@ struct Server::Slot {
   bool used = false;
 };
 
+struct Server::Lane {
+  hipStream_t st = nullptr;
+  std::vector<std::unique_ptr<Slot>> slots;
+  size_t next = 0;
+  size_t bytes = 0;  // resident bytes of its jobs
+};
+
 Server::Server(Context &ctx, double reserve_bytes, int max_jobs)
     : ctx_(ctx), max_jobs_(max_jobs), pool_(std::make_unique<Pool>()) {
   hip_check(hipSetDevice(ctx_.device), "hipSetDevice");
-  size_t freeb = 0, totalb = 0;
-  hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
-  const size_t reserve = reserve_bytes > 0 ? (size_t)reserve_bytes : (size_t)(4ull << 30);
-  budget_ = freeb > reserve ? freeb - reserve : freeb / 2;
+  reserve_ = reserve_bytes > 0 ? (size_t)reserve_bytes : (size_t)(4ull << 30);
   if (max_jobs_ <= 0 || max_jobs_ > 4096) max_jobs_ = 4096;
-  for (int i = 0; i < 3; ++i) {
-    auto s = std::make_unique<Slot>();
-    hip_check(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming), "hipEventCreate");
-    slots_.push_back(std::move(s));
+  for (int l = 0; l < kLanes; ++l) {
+    auto ln = std::make_unique<Lane>();
+    ln->st = ctx_.lane_stream(l);
+    for (int i = 0; i < kSlotsPerLane; ++i) {
+      auto s = std::make_unique<Slot>();
+      hip_check(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming), "hipEventCreate");
+      ln->slots.push_back(std::move(s));
+    }
+    lanes_.push_back(std::move(ln));
   }
   th_ = std::thread([this] { loop(); });
+}
+
+int Server::lane_of(int64_t N, bool f64) {
+  const auto key = std::make_pair(N, f64);
+  auto it = shape_lane_.find(key);
+  if (it != shape_lane_.end()) return it->second;
+  // a new shape takes the lane that holds the fewest shapes, then bytes
+  std::vector<int> shapes(lanes_.size(), 0);
+  for (auto &kv : shape_lane_) shapes[(size_t)kv.second]++;
+  int best = 0;
+  for (int l = 1; l < (int)lanes_.size(); ++l)
+    if (shapes[l] < shapes[best] || (shapes[l] == shapes[best] && lanes_[l]->bytes < lanes_[best]->bytes)) best = l;
+  shape_lane_[key] = best;
+  return best;
 }
 
 Server::~Server() {
@@ -128,15 +181,16 @@ Server::~Server() {
   }
   cv_.notify_all();
   if (th_.joinable()) th_.join();
-  (void)hipStreamSynchronize(ctx_.stream);
+  for (auto &ln : lanes_) (void)hipStreamSynchronize(ln->st);
   for (auto &j : active_) {
     if (j->done_ev) (void)hipEventDestroy(j->done_ev);
   }
-  for (auto &s : slots_) {
-    if (s->ev) (void)hipEventDestroy(s->ev);
-    if (s->host) (void)hipHostFree(s->host);
-    if (s->dev) (void)hipFree(s->dev);
-  }
+  for (auto &ln : lanes_)
+    for (auto &s : ln->slots) {
+      if (s->ev) (void)hipEventDestroy(s->ev);
+      if (s->host) (void)hipHostFree(s->host);
+      if (s->dev) (void)hipFree(s->dev);
+    }
   for (auto &kv : reqs_)
     if (kv.second->bounds) (void)hipFree(kv.second->bounds);
   pool_.reset();
@@ -247,6 +301,7 @@ void Server::admit() {
       rid = pending_.front();
       r = reqs_[rid];
     }
+    bool failed = false;
     if (!r->bounds) {
       const SimConfig &c = r->r.members[0];
       r->bounds_host = detector_bounds(c.N, c.dx, c.detector_size, c.px);
@@ -268,16 +323,44 @@ void Server::admit() {
       for (auto &o : j->m.ops) stepped |= o.kind == OpKind::Restore;
       j->field_bytes = (size_t)c.N * (c.f64 ? 16 : 8);
       j->acc_bytes = (size_t)ns * std::max<int64_t>(r->res.n_px, 1) * sizeof(double);
+      std::vector<size_t> sizes{j->field_bytes, j->acc_bytes};
+      if (stepped) sizes.push_back(j->field_bytes);
       const size_t need = j->field_bytes * (stepped ? 2 : 1) + j->acc_bytes;
-      size_t in_use = 0;
-      for (auto &a : active_) in_use += a->bytes();
-      if (in_use + need > budget_) return;  // wait for running jobs to finish
-      if (pool.dev_bytes + need > budget_) pool.trim(budget_, need);
+      // idle buffers of the right sizes are reused; the rest must come from
+      // free device memory plus the other idle buffers (released on demand),
+      // less the reserve
+      const size_t reuse = pool.reusable(sizes);
+      const size_t fresh = need - reuse;
+      size_t freeb = 0, totalb = 0;
+      hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
+      const size_t other_idle = pool.idle_bytes - reuse;
+      const size_t avail = freeb + other_idle > reserve_ ? freeb + other_idle - reserve_ : 0;
+      if (fresh > avail) {
+        if (!active_.empty()) return;  // wait for running jobs to finish
+        // nothing else is resident: this job can never fit
+        std::lock_guard<std::mutex> lk(mu_);
+        r->res.state = 3;
+        r->res.error = "request needs " + std::to_string(need >> 20) + " MiB per energy, " +
+                       std::to_string((avail + reuse) >> 20) + " MiB can be allocated (RS_ERR_NOMEM)";
+        r->res.done_t = now_s();
+        r->member_rows.clear();
+        done_cv_.notify_all();
+        failed = true;
+        break;
+      }
+      if (fresh > 0 && freeb < fresh + reserve_) {
+        // release idle buffers of other sizes until the fresh part fits
+        const size_t drop = fresh + reserve_ - freeb;
+        pool.trim(pool.idle_bytes > drop ? pool.idle_bytes - drop : 0, sizes);
+      }
+      j->lane = lane_of(c.N, c.f64);
+      j->pace = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxPassesPerTick, kPaceSamples / c.N));
+      hipStream_t st = lanes_[(size_t)j->lane]->st;
       j->field = pool.get_dev(j->field_bytes);
       if (stepped) j->snap = pool.get_dev(j->field_bytes);
       j->acc = (double *)pool.get_dev(j->acc_bytes);
       j->host_acc = (double *)pool.get_host(j->acc_bytes);
-      hip_check(launch_fill_f64(j->acc, (int64_t)(j->acc_bytes / sizeof(double)), 0.0, ctx_.stream), "acc fill");
+      hip_check(launch_fill_f64(j->acc, (int64_t)(j->acc_bytes / sizeof(double)), 0.0, st), "acc fill");
       hip_check(hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming), "hipEventCreate");
       if (c.f64) compile_job<double>(*j);
       else compile_job<float>(*j);
@@ -289,17 +372,18 @@ void Server::admit() {
           r->res.start_t = now_s();
         }
       }
+      lanes_[(size_t)j->lane]->bytes += j->bytes();
       active_.push_back(std::move(j));
       stats_.max_active = std::max<int64_t>(stats_.max_active, (int64_t)active_.size());
     }
     std::lock_guard<std::mutex> lk(mu_);
     pending_.pop_front();
+    (void)failed;
   }
 }
 
 bool Server::tick() {
   using exec::LKind;
-  hipStream_t st = ctx_.stream;
   struct Key {
     int kind;
     bool f64;
@@ -309,121 +393,160 @@ bool Server::tick() {
     int top;
     int xv;
     int64_t S;
+    int64_t span_off, span_len;
     const void *tab;
     bool operator<(const Key &o) const {
-      return std::tie(kind, f64, N, lgL, row, top, xv, S, tab) <
-             std::tie(o.kind, o.f64, o.N, o.lgL, o.row, o.top, o.xv, o.S, o.tab);
+      return std::tie(kind, f64, N, lgL, row, top, xv, S, span_off, span_len, tab) <
+             std::tie(o.kind, o.f64, o.N, o.lgL, o.row, o.top, o.xv, o.S, o.span_off, o.span_len, o.tab);
     }
   };
   struct Group {
     std::vector<Job *> jobs;
-    const exec::Launch *l = nullptr;
+    std::vector<size_t> at;  // launch index of each job
     size_t off = 0;
   };
-  std::map<Key, Group> groups;
-  bool any = false;
-  for (auto &jp : active_) {
-    Job &j = *jp;
-    if (j.cursor >= j.prog.size()) continue;
-    any = true;
-    const exec::Launch &l = j.prog[j.cursor];
-    if (l.kind == LKind::Copy || l.kind == LKind::Fill) {
-      hip_check(j.f64 ? launch_copy<double>(l.dst, l.src, l.n, st) : launch_copy<float>(l.dst, l.src, l.n, st),
-                "copy");
-      stats_.launches++;
-      stats_.merged_jobs++;
-      continue;
+  bool issued = false;
+  for (size_t li = 0; li < lanes_.size(); ++li) {
+    Lane &ln = *lanes_[li];
+    std::vector<Job *> js;
+    int reps = 0;
+    for (auto &jp : active_)
+      if (jp->lane == (int)li && jp->cursor < jp->prog.size()) {
+        js.push_back(jp.get());
+        reps = std::max(reps, jp->pace);
+      }
+    if (js.empty()) continue;
+    Slot &s = *ln.slots[ln.next];
+    if (s.used) {
+      if (hipEventQuery(s.ev) == hipErrorNotReady) continue;  // every slot of the lane is in flight
+      hip_check(hipEventSynchronize(s.ev), "slot wait");
     }
-    Key k{(int)l.kind, j.f64, j.N, l.lgL, l.row, l.geom.top, l.xv, l.geom.S,
-          l.kind == LKind::Detect ? (const void *)j.req->bounds : l.geom.tw_fib};
-    Group &g = groups[k];
-    g.jobs.push_back(&j);
-    g.l = &l;
-  }
-  if (!any) return false;
-  // stage all argument records of this tick in one slot
-  Slot &s = *slots_[slot_i_];
-  slot_i_ = (slot_i_ + 1) % slots_.size();
-  if (s.used) hip_check(hipEventSynchronize(s.ev), "slot wait");
-  size_t need = 0;
-  for (auto &kv : groups)
-    need += kv.second.jobs.size() * (kv.first.kind == (int)LKind::Detect ? sizeof(DetArgs) : sizeof(FieldArgs)) + 64;
-  if (need > s.cap) {
-    if (s.host) (void)hipHostFree(s.host);
-    if (s.dev) (void)hipFree(s.dev);
-    s.cap = std::max(need * 2, (size_t)1 << 16);
-    hip_check(hipHostMalloc(&s.host, s.cap, hipHostMallocDefault), "hipHostMalloc(args)");
-    hip_check(hipMalloc(&s.dev, s.cap), "hipMalloc(args)");
-  }
-  size_t off = 0;
-  for (auto &kv : groups) {
-    Group &g = kv.second;
-    off = (off + 63) & ~(size_t)63;
-    g.off = off;
-    for (Job *j : g.jobs) {
-      const exec::Launch &l = j->prog[j->cursor];
-      if (kv.first.kind == (int)LKind::Detect) {
-        std::memcpy((char *)s.host + off, &j->dargs[l.arg_off], sizeof(DetArgs));
-        off += sizeof(DetArgs);
-      } else {
-        std::memcpy((char *)s.host + off, &j->fargs[l.arg_off], sizeof(FieldArgs));
-        off += sizeof(FieldArgs);
+    ln.next = (ln.next + 1) % ln.slots.size();
+    hipStream_t st = ln.st;
+    // sub-step r: the r-th next pass of every job that advances that far
+    // this tick; copies and fills run on their own, the rest merge by kernel
+    std::vector<std::map<Key, Group>> steps;
+    std::vector<std::vector<std::pair<Job *, size_t>>> singles;
+    for (int r = 0; r < reps; ++r) {
+      std::map<Key, Group> groups;
+      std::vector<std::pair<Job *, size_t>> one;
+      for (Job *j : js) {
+        const size_t c = j->cursor + (size_t)r;
+        if (r >= j->pace || c >= j->prog.size()) continue;
+        const exec::Launch &l = j->prog[c];
+        if (l.kind == LKind::Copy || l.kind == LKind::Fill) {
+          one.emplace_back(j, c);
+          continue;
+        }
+        Key k{(int)l.kind, j->f64, j->N, l.lgL, l.row, l.geom.top, l.xv, l.geom.S, l.geom.span_off, l.geom.span_len,
+              l.kind == LKind::Detect ? (const void *)j->req->bounds : l.geom.tw_fib};
+        Group &g = groups[k];
+        g.jobs.push_back(j);
+        g.at.push_back(c);
+      }
+      if (groups.empty() && one.empty()) break;
+      steps.push_back(std::move(groups));
+      singles.push_back(std::move(one));
+    }
+    // stage the argument records of every merged launch in the lane's slot
+    size_t need = 0;
+    for (auto &groups : steps)
+      for (auto &kv : groups)
+        need += kv.second.jobs.size() * (kv.first.kind == (int)LKind::Detect ? sizeof(DetArgs) : sizeof(FieldArgs)) +
+                64;
+    if (need > s.cap) {
+      if (s.host) (void)hipHostFree(s.host);
+      if (s.dev) (void)hipFree(s.dev);
+      s.host = s.dev = nullptr;
+      s.cap = 0;
+      const size_t cap = std::max(need * 2, (size_t)1 << 16);
+      hip_check(hipHostMalloc(&s.host, cap, hipHostMallocDefault), "hipHostMalloc(args)");
+      hip_check(hipMalloc(&s.dev, cap), "hipMalloc(args)");
+      s.cap = cap;
+    }
+    size_t off = 0;
+    for (auto &groups : steps)
+      for (auto &kv : groups) {
+        Group &g = kv.second;
+        off = (off + 63) & ~(size_t)63;
+        g.off = off;
+        for (size_t i = 0; i < g.jobs.size(); ++i) {
+          const exec::Launch &l = g.jobs[i]->prog[g.at[i]];
+          if (kv.first.kind == (int)LKind::Detect) {
+            std::memcpy((char *)s.host + off, &g.jobs[i]->dargs[l.arg_off], sizeof(DetArgs));
+            off += sizeof(DetArgs);
+          } else {
+            std::memcpy((char *)s.host + off, &g.jobs[i]->fargs[l.arg_off], sizeof(FieldArgs));
+            off += sizeof(FieldArgs);
+          }
+        }
+      }
+    if (off) hip_check(hipMemcpyAsync(s.dev, s.host, off, hipMemcpyHostToDevice, st), "args upload");
+    for (size_t r = 0; r < steps.size(); ++r) {
+      for (auto &jc : singles[r]) {
+        const exec::Launch &l = jc.first->prog[jc.second];
+        if (l.kind == LKind::Fill)
+          hip_check(launch_fill_f64((double *)l.dst, l.n, 0.0, st), "fill");
+        else
+          hip_check(jc.first->f64 ? launch_copy<double>(l.dst, l.src, l.n, st)
+                                  : launch_copy<float>(l.dst, l.src, l.n, st),
+                    "copy");
+        stats_.launches++;
+        stats_.merged_jobs++;
+      }
+      for (auto &kv : steps[r]) {
+        Group &g = kv.second;
+        const int nf = (int)g.jobs.size();
+        const char *dp = (const char *)s.dev + g.off;
+        const exec::Launch &l = g.jobs[0]->prog[g.at[0]];
+        const bool f64 = kv.first.f64;
+        switch (l.kind) {
+          case LKind::Fiber:
+            hip_check(f64 ? launch_fiber_pass<double>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st)
+                          : launch_fiber_pass<float>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st),
+                      "fiber pass");
+            break;
+          case LKind::Xop:
+            hip_check(f64 ? launch_xop<double>((const FieldArgs *)dp, nf, kv.first.N, st)
+                          : launch_xop<float>((const FieldArgs *)dp, nf, kv.first.N, st),
+                      "xop");
+            break;
+          case LKind::Kop:
+            hip_check(f64 ? launch_kop<double>((const FieldArgs *)dp, nf, l.geom, st)
+                          : launch_kop<float>((const FieldArgs *)dp, nf, l.geom, st),
+                      "kop");
+            break;
+          case LKind::Detect: {
+            Job *j0 = g.jobs[0];
+            hip_check(f64 ? launch_detect<double>((const DetArgs *)dp, nf, (const int64_t *)j0->req->bounds,
+                                                  j0->req->res.n_px, st)
+                          : launch_detect<float>((const DetArgs *)dp, nf, (const int64_t *)j0->req->bounds,
+                                                 j0->req->res.n_px, st),
+                      "detect");
+            break;
+          }
+          default:
+            throw Error(6, "unsupported launch kind in serving");
+        }
+        stats_.launches++;
+        stats_.merged_jobs += nf;
       }
     }
-  }
-  hip_check(hipMemcpyAsync(s.dev, s.host, off, hipMemcpyHostToDevice, st), "args upload");
-  for (auto &kv : groups) {
-    Group &g = kv.second;
-    const int nf = (int)g.jobs.size();
-    const char *dp = (const char *)s.dev + g.off;
-    const exec::Launch &l = *g.l;
-    const bool f64 = kv.first.f64;
-    switch (l.kind) {
-      case LKind::Fiber:
-        hip_check(f64 ? launch_fiber_pass<double>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st)
-                      : launch_fiber_pass<float>(l.lgL, l.row, l.xv, (const FieldArgs *)dp, nf, l.geom, st),
-                  "fiber pass");
-        break;
-      case LKind::Xop:
-        hip_check(f64 ? launch_xop<double>((const FieldArgs *)dp, nf, kv.first.N, st)
-                      : launch_xop<float>((const FieldArgs *)dp, nf, kv.first.N, st),
-                  "xop");
-        break;
-      case LKind::Kop:
-        hip_check(f64 ? launch_kop<double>((const FieldArgs *)dp, nf, l.geom, st)
-                      : launch_kop<float>((const FieldArgs *)dp, nf, l.geom, st),
-                  "kop");
-        break;
-      case LKind::Detect: {
-        Job *j0 = g.jobs[0];
-        hip_check(f64 ? launch_detect<double>((const DetArgs *)dp, nf, (const int64_t *)j0->req->bounds,
-                                              j0->req->res.n_px, st)
-                      : launch_detect<float>((const DetArgs *)dp, nf, (const int64_t *)j0->req->bounds,
-                                             j0->req->res.n_px, st),
-                  "detect");
-        break;
+    hip_check(hipEventRecord(s.ev, st), "slot event");
+    s.used = true;
+    issued = true;
+    // advance the lane's jobs; finished jobs read back their rows
+    for (Job *j : js) {
+      j->cursor = std::min(j->prog.size(), j->cursor + (size_t)j->pace);
+      if (j->cursor == j->prog.size() && !j->enqueued_all) {
+        hip_check(hipMemcpyAsync(j->host_acc, j->acc, j->acc_bytes, hipMemcpyDeviceToHost, st), "acc readback");
+        hip_check(hipEventRecord(j->done_ev, st), "done event");
+        j->enqueued_all = true;
       }
-      default:
-        throw Error(6, "unsupported launch kind in serving");
     }
-    stats_.launches++;
-    stats_.merged_jobs += nf;
+    stats_.ticks++;
   }
-  hip_check(hipEventRecord(s.ev, st), "slot event");
-  s.used = true;
-  // advance every job by one pass; finished jobs read back their rows
-  for (auto &jp : active_) {
-    Job &j = *jp;
-    if (j.cursor >= j.prog.size()) continue;
-    j.cursor++;
-    if (j.cursor == j.prog.size() && !j.enqueued_all) {
-      hip_check(hipMemcpyAsync(j.host_acc, j.acc, j.acc_bytes, hipMemcpyDeviceToHost, st), "acc readback");
-      hip_check(hipEventRecord(j.done_ev, st), "done event");
-      j.enqueued_all = true;
-    }
-  }
-  stats_.ticks++;
-  return true;
+  return issued;
 }
 
 void Server::reap() {
@@ -436,7 +559,8 @@ void Server::reap() {
     }
     auto r = j.req;
     const size_t n = j.acc_bytes / sizeof(double);
-    r->member_rows[j.member].assign(j.host_acc, j.host_acc + n);
+    if (r->res.state != 3) r->member_rows[j.member].assign(j.host_acc, j.host_acc + n);
+    lanes_[(size_t)j.lane]->bytes -= j.bytes();
     pool.put_dev(j.field_bytes, j.field);
     if (j.snap) pool.put_dev(j.field_bytes, j.snap);
     pool.put_dev(j.acc_bytes, j.acc);
@@ -444,7 +568,7 @@ void Server::reap() {
     (void)hipEventDestroy(j.done_ev);
     r->done++;
     stats_.jobs_done++;
-    if (r->done == r->r.members.size()) {
+    if (r->done == r->r.members.size() && r->res.state != 3) {
       // weighted sum in member order, then the detector geometry factors
       const SimConfig &c = r->r.members[0];
       const int64_t npx = r->res.n_px, ns = r->res.n_steps;
@@ -482,11 +606,28 @@ void Server::loop() {
     try {
       const double t0 = now_s();
       admit();
-      const bool worked = tick();
+      const bool issued = tick();
       reap();
-      if (!worked && !active_.empty()) {
-        // every admitted job is waiting for its readback
-        (void)hipStreamSynchronize(ctx_.stream);
+      if (!issued && !active_.empty()) {
+        // every lane with work has all of its slots in flight, or every job
+        // waits for its readback: block on the oldest of that work
+        bool waited = false;
+        for (size_t li = 0; li < lanes_.size() && !waited; ++li) {
+          Slot &s = *lanes_[li]->slots[lanes_[li]->next];
+          bool busy = false;
+          for (auto &jp : active_) busy |= jp->lane == (int)li && jp->cursor < jp->prog.size();
+          if (busy && s.used) {
+            hip_check(hipEventSynchronize(s.ev), "slot wait");
+            waited = true;
+          }
+        }
+        if (!waited) {
+          for (auto &jp : active_)
+            if (jp->enqueued_all) {
+              hip_check(hipEventSynchronize(jp->done_ev), "job wait");
+              break;
+            }
+        }
         reap();
       }
       stats_.busy_s += now_s() - t0;

@@ -4,16 +4,23 @@
 // one or more energies.  Each (request, energy) pair is a *job*: one member
 // program (simulation.cpp build_member) with its own field, snapshot and
 // detector rows in device memory.  Jobs are admitted in arrival order while
-// their memory fits the device budget (hipMemGetInfo at start minus a
-// reserve), so fields stay resident from admission to completion.
+// their memory fits: free device memory now (hipMemGetInfo, read at every
+// admission) plus the scheduler's idle cached buffers, minus a reserve.  A
+// job that cannot fit even with nothing else resident fails its request
+// (RS_ERR_NOMEM) instead of blocking the queue.  Fields stay resident from
+// admission to completion.
 //
-// The scheduler thread advances every admitted job by one pass per tick and
-// merges, per tick, the passes of all jobs that need the same kernel (same
-// plan, pass kind and fiber length) into one launch (blockIdx.y = job).  A job
-// admitted while others are mid-way simply joins the next tick's launches.
-// Every job writes its own detector rows; a request's result is the weighted
-// sum over its jobs taken on the host in energy order, so it does not depend
-// on which other requests shared the launches.
+// Jobs run on lanes: each grid shape (N, precision) is given one of up to
+// kLanes streams, so jobs of different sizes overlap on the device.  On every
+// tick the scheduler gives each lane whose argument slots are free its next
+// passes: a job of N samples advances by about 2^26 / N passes (1 to 64), so
+// small fields are not paced by the launches of large ones, and the passes of
+// all jobs of a lane that need the same kernel (same plan, pass kind, fiber
+// length and span) merge into one launch (blockIdx.y = job).  A job admitted
+// while others are mid-way simply joins the next tick's launches.  Every job
+// writes its own detector rows; a request's result is the weighted sum over
+// its jobs taken on the host in energy order, so it does not depend on which
+// other requests shared the launches.
 #pragma once
 #include <atomic>
 #include <condition_variable>
@@ -69,11 +76,14 @@ class Server {
   void reap();
   template <typename S> void compile_job(Job &j);
 
+  // lane of a grid shape (assigned on first sight, fewest resident bytes)
+  int lane_of(int64_t N, bool f64);
+
   Context &ctx_;
   int max_jobs_;
   struct Pool;
   std::unique_ptr<Pool> pool_;
-  size_t budget_ = 0;
+  size_t reserve_ = 0;
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
   std::deque<int64_t> pending_;
@@ -83,10 +93,11 @@ class Server {
   bool stop_ = false;
   ServeStats stats_;
   std::thread th_;
-  // args staging ring
+  // per lane: its stream and a ring of argument staging slots
   struct Slot;
-  std::vector<std::unique_ptr<Slot>> slots_;
-  size_t slot_i_ = 0;
+  struct Lane;
+  std::vector<std::unique_ptr<Lane>> lanes_;
+  std::map<std::pair<int64_t, bool>, int> shape_lane_;
 };
 
 }  // namespace rs

@@ -7,8 +7,9 @@ saved u vectors (multisim.py:404-462), the save_and_exit cut
 (multisim.py:166-192), the Nyquist check of the analytic source
 (propagation.py:31-45), cutoff angles from the detector / element geometry
 (propagation.py:178-276) and the reflection check on N (multisim.py:480-487).
-Delta/beta values are supplied by the caller (a table or a function of
-(material, density, energy)) instead of being looked up in nist_lookup.
+Delta/beta values come from ravesim.xray (the Chantler tables, as big-wave's
+nist_lookup call at multisim.py:387 gives them) unless the caller supplies a
+table or a function of (material, density, energy).
 """
 from __future__ import annotations
 
@@ -88,8 +89,10 @@ def _thickness(el: dict, config_dir: Optional[Path]) -> float:
     return float(el["thickness"]) + float(el.get("substrate_thickness", 0.0))
 
 
-def _table(materials, energy: float, deltabeta: DeltaBeta) -> list:
+def _table(materials, energy: float, deltabeta: Optional[DeltaBeta]) -> list:
     rows = []
+    if deltabeta is None:
+        from .xray import xray_delta_beta as deltabeta
     for name, dens in materials:
         if callable(deltabeta):
             d, b = deltabeta(name, dens, energy)
@@ -202,7 +205,7 @@ def _reduce_for_save_and_exit(cfg: dict, max_x_list: Sequence[float]) -> float:
     return max_x
 
 
-def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBeta,
+def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: Optional[DeltaBeta] = None,
                      config_dir: Optional[Union[str, Path]] = None,
                      spectrum: Optional[Tuple[np.ndarray, np.ndarray]] = None) -> Path:
     """Set up `sim_dir` from a big-wave config (big-wave/multisim.py:282-521).
@@ -214,7 +217,9 @@ def setup_simulation(config: dict, sim_dir: Union[str, Path], deltabeta: DeltaBe
     vector source per source of `base_sim_dir`, reading its
     u_{input_u_index:04}.npy at the base detector's z.  A save_and_exit element
     ends the setup there and turns on save_final_u_vectors.  Sample grids are
-    read relative to `config_dir` and copied into the sim dir.
+    read relative to `config_dir` and copied into the sim dir.  `deltabeta`
+    None: every material's delta and beta at each source energy from
+    ravesim.xray.xray_delta_beta (big-wave/multisim.py:387).
     """
     import random
 

@@ -193,3 +193,31 @@ def random_case(tmp: Path, seed: int):
              "deltabeta_table": [[["Si", 2.34], [5.1e-7, 1.6e-10]], [["Au", 19.32], [3.6e-6, 2.1e-7]]]}]
     ang = sorted(float(rng.choice([float(rng.uniform(0.2e-3, 2e-3)), math.pi / 2])) for _ in range(n_el + 1))
     return cfg, subs, ang, grid
+
+
+def sphere_grid(radius: float, px: float, pz: float, seed: int = 0, pitch: float = 4.2e-6):
+    """Sphere phantom as a sample grid: material 1 where the pixel centre
+    ((ix + 1/2 - nx/2) px - s, (iz + 1/2 - nz/2) pz) lies within `radius` of
+    the centre, 0 elsewhere; the lateral offset s of the centre is uniform
+    within one grating pitch from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    s = float(rng.uniform(-pitch / 2, pitch / 2))
+    nx = int(round(2 * radius / px)) + 2
+    nz = int(round(2 * radius / pz))
+    x = (np.arange(nx) + 0.5 - nx / 2) * px - s
+    z = (np.arange(nz) + 0.5 - nz / 2) * pz
+    return (x[None, :] ** 2 + z[:, None] ** 2 <= radius * radius).astype(np.int8)
+
+
+def talbot_lau_phantom(tmp: Path, N=2 ** 26, slices=64, det_size=None, dtype="c8", pz=1e-6, seed=0):
+    """The Talbot-Lau chain with its optional sample: a sphere of radius
+    50 um, delta = 3e-7, beta = 0, on 1 um x pixels and `pz` z pixels,
+    starting 0.1 m behind G1 (100 extra propagation steps at pz = 1 um)."""
+    cfg, subs, ang = talbot_lau(tmp, N=N, slices=slices, det_size=det_size, dtype=dtype)
+    grid = sphere_grid(50e-6, 1e-6, pz, seed)
+    np.save(tmp / "sphere.npy", grid)
+    sample = {"type": "sample", "z_start": 0.3, "pixel_size_x": 1e-6, "pixel_size_z": pz,
+              "grid_path": "sphere.npy", "materials": [["sphere", 1.0]], "x_positions": [0.0]}
+    cfg["elements"].insert(2, sample)
+    subs[0]["deltabeta_table"].append([["sphere", 1.0], [3e-7, 0.0]])
+    return cfg, subs, [math.pi / 2] * 5

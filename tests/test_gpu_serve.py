@@ -71,25 +71,66 @@ def test_alone_and_in_the_mix_are_identical(server):
     assert np.array_equal(alone, mixed)
 
 
-def test_admission_with_small_budget_completes():
-    """A budget that holds one job at a time still serves every request."""
+def _free_device_bytes():
     import ctypes as C
-    import ravesim
-    from ravesim.serve import Server, phase_sweep
-    ctx = ravesim.Context(0)
     free = C.c_size_t()
     total = C.c_size_t()
     lib = C.CDLL("libamdhip64.so")
     lib.hipMemGetInfo(C.byref(free), C.byref(total))
-    one_job = 2 * 2 ** 18 * 8 + 8 * 100 * 8
-    s = Server(ctx, reserve_bytes=float(free.value - int(1.5 * one_job)))
-    reqs = [phase_sweep(2 ** 18, [46000.0 + 500 * i], n_steps=2, slices=2) for i in range(3)]
+    return free.value
+
+
+def test_admission_with_small_budget_completes():
+    """A budget that holds one job at a time still serves every request:
+    the second job waits for the first one's buffers and reuses them."""
+    import ravesim
+    from ravesim.serve import Server, phase_sweep
+    ctx = ravesim.Context(0)
+    one_job = 2 * 2 ** 22 * 8 + 8 * 500 * 8  # field + snapshot + detector rows
+    s = Server(ctx, reserve_bytes=float(_free_device_bytes() - int(1.5 * one_job)))
+    reqs = [phase_sweep(2 ** 22, [46000.0 + 500 * i], n_steps=2, slices=2) for i in range(3)]
     ids = [s.submit(r) for r in reqs]
     outs = [s.result(i, timeout_s=120).rows for i in ids]
     npx = int(reqs[0].det_size // reqs[0].det_px)
     assert all(np.isfinite(o).all() and o.shape == (2, npx) for o in outs)
     assert s.stats()["max_resident_jobs"] == 1
     s.close()
+    ctx.close()
+
+
+def test_job_that_can_never_fit_fails_its_request():
+    """With a reserve above the device's memory no job fits even alone: the
+    request fails with a memory message instead of waiting forever, later
+    requests are refused the same way, and the server shuts down."""
+    import ravesim
+    from ravesim._lib import RsError
+    from ravesim.serve import Server, phase_sweep
+    ctx = ravesim.Context(0)
+    s = Server(ctx, reserve_bytes=float(_free_device_bytes()) * 4)
+    t0 = time.perf_counter()
+    ids = [s.submit(phase_sweep(2 ** 16, [46000.0], n_steps=2, slices=2)) for _ in range(2)]
+    for i in ids:
+        with pytest.raises(RsError, match="RS_ERR_NOMEM"):
+            s.result(i, timeout_s=60)
+    s.close()
+    assert time.perf_counter() - t0 < 30
+    ctx.close()
+
+
+def test_context_is_reserved_while_a_server_lives():
+    import ravesim
+    from ravesim._lib import RsError
+    from ravesim.serve import Server
+    ctx = ravesim.Context(0)
+    f = ravesim.Field(ctx, 2 ** 12, 2e-9)
+    s = Server(ctx)
+    for call in (lambda: f.propagate(2.7e-11, 1e-3), lambda: ravesim.Field(ctx, 2 ** 12, 2e-9),
+                 lambda: ctx.autotune(2 ** 12), lambda: Server(ctx)):
+        with pytest.raises(RsError, match="reserved by a server"):
+            call()
+    s.close()
+    f.propagate(2.7e-11, 1e-3)  # usable again
+    f.close()
     ctx.close()
 
 

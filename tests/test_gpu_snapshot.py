@@ -1,12 +1,10 @@
 """big-wave's end-to-end golden vector (big-wave/test.py:853-960) through the
-product path: sim dir written by ravesim.simdir.setup_simulation (delta/beta
-from the oracle's Chantler restatement), run by rs_run_simdir on the GPU."""
+product path: sim dir written by ravesim.simdir.setup_simulation with its own
+delta/beta (ravesim.xray, the default), run by rs_run_simdir on the GPU."""
 import json
 
 import numpy as np
 import pytest
-
-from oracle import xray
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +17,7 @@ def test_golden_vector_on_gpu(golden_dir, tmp_path, dtype, atol):
     d = json.loads((golden_dir / "bigwave_snapshot.json").read_text())
     cfg = dict(d["known_config"])
     cfg["dtype"] = dtype
-    sd = setup_simulation(cfg, tmp_path / "sim", lambda m, rho, e: xray.xray_delta_beta(m, rho, e))
+    sd = setup_simulation(cfg, tmp_path / "sim")
     ctx = ravesim.Context(0)
     got = run_single_simulation(sd, 0, tmp_path, ctx=ctx)[0]
     exp = np.array(d["known_data"])
