@@ -220,6 +220,19 @@ int rs_scale(rs_field *f, double re, double im);
 int rs_square_and_downsample(rs_field *f, const rs_detector *det, double z, double *out,
                              int64_t n_out);
 
+/* Allocation-free variant of rs_square_and_downsample, with the same
+ * results bit for bit.  It has no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid hipMalloc): it calls no hipMalloc / hipFree and allocates no
+ * host memory.  The pixel bounds and FP64 sums go to a fixed scratch array of
+ * 1 MiB per GPU (held until the process ends, its address looked up when a
+ * context is created), so n_px may be at most RS_NOALLOC_MAX_PIXELS (larger
+ * detectors: RS_ERR_INVALID).  Every call on one GPU uses that GPU's array:
+ * calls that overlap on one GPU, from different contexts or threads, race. */
+#define RS_NOALLOC_MAX_PIXELS 65536
+int rs_square_and_downsample_noalloc(rs_field *f, const rs_detector *det, double z, double *out,
+                                     int64_t n_out);
+
 /* Number of detector pixels for `det` (int(size // pixel_x)). */
 int64_t rs_detector_pixels(const rs_detector *det);
 

@@ -16,6 +16,8 @@
 #include "simulation.hpp"
 #include "serve.hpp"
 
+static_assert(RS_NOALLOC_MAX_PIXELS == rs::kNoAllocMaxPixels, "allocation-free detector limit");
+
 struct rs_ctx {
   std::unique_ptr<rs::Context> ctx;
   // servers attached to the context: while one lives, its scheduler thread
@@ -482,6 +484,36 @@ int rs_square_and_downsample(rs_field *f, const rs_detector *det, double z, doub
     rc.z_det = z;
     run_on_field(f, m, rc, out, n_out);
   });
+}
+
+int rs_square_and_downsample_noalloc(rs_field *f, const rs_detector *det, double z, double *out, int64_t n_out) {
+  if (!f || !det || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(f->owner)) return fail_reserved();
+  const int64_t npx = rs::detector_pixels(det->size, det->pixel_x);
+  if (npx > RS_NOALLOC_MAX_PIXELS)
+    return fail(RS_ERR_INVALID, "detector has " + std::to_string(npx) + " pixels, more than the " +
+                                    std::to_string(RS_NOALLOC_MAX_PIXELS) + " of the allocation-free read");
+  if (npx > f->grid.N) return fail(RS_ERR_INVALID, "detector has more pixels than samples");
+  if (n_out < npx) return fail(RS_ERR_INVALID, "output buffer too small");
+  if (npx <= 0) return RS_OK;
+  rs::Context &c = *f->owner->ctx;
+  const hipError_t e =
+      f->f64 ? rs::launch_detect_noalloc<double>(f->buf.p, f->grid.N, f->grid.dx, npx, det->pixel_x, c.noalloc_scr,
+                                                 c.stream)
+             : rs::launch_detect_noalloc<float>(f->buf.p, f->grid.N, f->grid.dx, npx, det->pixel_x, c.noalloc_scr,
+                                                c.stream);
+  if (e != hipSuccess) return fail(RS_ERR_DEVICE, std::string("detector kernels: ") + hipGetErrorString(e));
+  // the FP64 pixel sums straight into the caller's buffer, then big-wave's
+  // geometry factors in place: ((sum * cos) * dx * py) / dist
+  hipError_t c1 = hipMemcpyAsync(out, c.noalloc_scr, (size_t)npx * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (c1 == hipSuccess) c1 = hipStreamSynchronize(c.stream);
+  if (c1 != hipSuccess) return fail(RS_ERR_DEVICE, std::string("detector read-back: ") + hipGetErrorString(c1));
+  for (int64_t k = 0; k < npx; ++k) {
+    double cos_a, dist;
+    rs::detector_pixel_scale(det->size, det->pixel_x, z, k, &cos_a, &dist);
+    out[k] = (((out[k] * cos_a) * f->grid.dx) * det->pixel_y) / dist;
+  }
+  return RS_OK;
 }
 
 int rs_detector_bounds(int64_t N, double dx, const rs_detector *det, int64_t *bounds, int64_t n_bounds) {

@@ -17,6 +17,7 @@
 
 #include "exec.hpp"
 #include "npy.hpp"
+#include "rs_detector.hpp"
 #include "rs_kernels.hpp"
 
 #pragma STDC FP_CONTRACT OFF
@@ -77,52 +78,26 @@ std::vector<int64_t> detector_bounds(int64_t N, double dx, double size, double p
   const int64_t o = detector_pixels(size, px);
   std::vector<int64_t> b(o + 1, 0);
   if (o <= 0) return b;
-  const double first = (-((double)o / 2.0)) * px;
-  const double last = ((double)o / 2.0 + 1.0) * px;
-  const int64_t nb = o + 1;
-  // np.linspace(first, last, nb + 1, endpoint=True)
-  std::vector<double> edges(nb + 1);
-  const double delta = last - first;
-  const double step = delta / (double)nb;
-  for (int64_t i = 0; i <= nb; ++i) edges[i] = (double)i * step + first;
-  edges[nb] = last;
-  const double denom = last - first;
-  auto binval = [&](int64_t j) -> int64_t {
-    const double x = ((double)j - (double)N / 2.0) * dx;
-    if (x < first) return -1;
-    if (x > last) return nb + 1;
-    const double f = ((x - first) / denom) * (double)nb;
-    int64_t idx = (int64_t)f;
-    if (idx == nb) idx -= 1;
-    if (x < edges[idx]) idx -= 1;
-    if (x >= edges[idx + 1] && idx != nb - 1) idx += 1;
-    return idx;
-  };
-  for (int64_t k = 0; k <= o; ++k) {
-    int64_t lo = 0, hi = N;  // first j in [0, N] with binval(j) >= k
-    while (lo < hi) {
-      const int64_t mid = lo + (hi - lo) / 2;
-      if (binval(mid) >= k) hi = mid;
-      else lo = mid + 1;
-    }
-    b[k] = lo;
-  }
+  const DetGeom g = det_geom(N, dx, o, px);
+  for (int64_t k = 0; k <= o; ++k) b[k] = det_bound(g, k);
   return b;
+}
+
+void detector_pixel_scale(double size, double px, double z, int64_t k, double *cos_a, double *dist) {
+  // x = (k - n//2) px (util.py:78-82), cos(atan(x / z)) and sqrt(x^2 + z^2)
+  const double n = py_floordiv(size, px);
+  const double h = py_floordiv(n, 2.0);
+  const double x = ((double)k - h) * px;
+  *cos_a = std::cos(std::atan(x / z));
+  *dist = std::sqrt(x * x + z * z);
 }
 
 std::vector<double> detector_scale(double size, double px, double py, double dx, double z) {
   // returned as pairs (cos(atan(x/z)), sqrt(x^2 + z^2)) so that the caller
   // applies them in big-wave's order: ((out * cos) * dx * py) / sqrt(...)
-  const double n = py_floordiv(size, px);
-  const double h = py_floordiv(n, 2.0);
-  const int64_t o = (int64_t)n;
+  const int64_t o = (int64_t)py_floordiv(size, px);
   std::vector<double> s(2 * std::max<int64_t>(o, 0));
-  for (int64_t k = 0; k < o; ++k) {
-    const double x = ((double)k - h) * px;
-    const double ang = std::atan(x / z);
-    s[2 * k] = std::cos(ang);
-    s[2 * k + 1] = std::sqrt(x * x + z * z);
-  }
+  for (int64_t k = 0; k < o; ++k) detector_pixel_scale(size, px, z, k, &s[2 * k], &s[2 * k + 1]);
   (void)dx;
   (void)py;
   return s;
@@ -370,6 +345,7 @@ std::string Plan::describe() const {
 Context::Context(int dev) : device(dev) {
   hip_check(hipSetDevice(dev), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  noalloc_scr = noalloc_scratch(dev);
 }
 Context::~Context() {
   plans_.clear();

@@ -51,6 +51,9 @@ int64_t detector_pixels(double size, double px);
 // bins (big-wave/propagation.py:293-308).  bounds has n_px + 1 entries.
 std::vector<int64_t> detector_bounds(int64_t N, double dx, double size, double px);
 
+// cos(atan(x/z)) and sqrt(x^2 + z^2) of pixel k, x = (k - n//2) px
+void detector_pixel_scale(double size, double px, double z, int64_t k, double *cos_a, double *dist);
+
 // final per-pixel factor cos(atan(x/z)) dx py / sqrt(x^2 + z^2),
 // x = (k - n//2) px (big-wave/propagation.py:312-320, util.py:78-82)
 std::vector<double> detector_scale(double size, double px, double py, double dx, double z);
@@ -177,6 +180,9 @@ class Context {
   ~Context();
   int device;
   hipStream_t stream = nullptr;   // all work of the context's calls is ordered on it
+  // this device's scratch memory of the allocation-free detector read
+  // (noalloc_scratch, looked up when the context is created)
+  void *noalloc_scr = nullptr;
   // lane i of the lockstep runner (0 = `stream`; others created on first use
   // and joined into `stream` at the end of each run)
   hipStream_t lane_stream(int i);

@@ -15,9 +15,12 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 
 #include "device_ops.hpp"
+#include "engine.hpp"
 #include "fiber.hpp"
+#include "rs_detector.hpp"
 #include "rs_kernels.hpp"
 
 namespace rs {
@@ -101,6 +104,81 @@ __global__ void __launch_bounds__(256) detect_kernel(const DetArgs *__restrict__
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free detector read (rs_square_and_downsample_noalloc)
+// ---------------------------------------------------------------------------
+
+// Device memory of the allocation-free detector read: the FP64 pixel sums
+// (kNoAllocMaxPixels doubles), then the pixel bounds (kNoAllocMaxPixels + 1
+// int64).  One copy per GPU, kept until the process ends; host code reaches
+// it only through the address noalloc_scratch() looked up.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(1, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    hip_check(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != device) throw Error(1, "noalloc_scratch: device is not current");
+    void *p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+// bounds[k] for k <= npx (one thread each, binary search over the samples)
+__global__ void __launch_bounds__(256) det_bounds_kernel(DetGeom g, int64_t npx, int64_t *__restrict__ bounds) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k <= npx) bounds[k] = det_bound(g, k);
+}
+
+// acc[k] = sum of |u_j|^2 over bounds[k] <= j < bounds[k + 1] in FP64, one
+// workgroup per pixel: the same sums, in the same order, as detect_kernel
+// for one field written into zeroed accumulators with weight 1
+template <typename S>
+__global__ void __launch_bounds__(256) detect_one_kernel(const void *__restrict__ data, double *__restrict__ acc,
+                                                         const int64_t *__restrict__ bounds) {
+  using T2 = typename C2T<S>::t;
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  const int64_t lo = bounds[k], hi = bounds[k + 1];
+  const T2 *u = (const T2 *)data;
+  double s = 0.0;
+  for (int64_t j = lo + threadIdx.x; j < hi; j += 256) {
+    const T2 z = u[j];
+    s += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+#pragma unroll
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[k] = 0.0 + 1.0 * red[0];
+}
+
+template <typename S>
+hipError_t launch_detect_noalloc(const void *data, int64_t N, double dx, int64_t npx, double px, void *scr,
+                                 hipStream_t st) {
+  if (npx <= 0) return hipSuccess;
+  if (npx > kNoAllocMaxPixels || npx > N) return hipErrorInvalidValue;
+  double *acc = (double *)scr;
+  int64_t *bounds = (int64_t *)((char *)scr + kNoAllocMaxPixels * sizeof(double));
+  const DetGeom g = det_geom(N, dx, npx, px);
+  hipLaunchKernelGGL(det_bounds_kernel, dim3((unsigned)((npx + 1 + 255) / 256)), dim3(256), 0, st, g, npx, bounds);
+  hipLaunchKernelGGL(detect_one_kernel<S>, dim3((unsigned)npx), dim3(256), 0, st, data, acc,
+                     (const int64_t *)bounds);
+  return hipGetLastError();
 }
 
 __global__ void fill_f64_kernel(double *p, int64_t n, double v) {
@@ -188,6 +266,10 @@ template hipError_t launch_kop<double>(const FieldArgs *, int, const Geom &, hip
 template hipError_t launch_detect<float>(const DetArgs *, int, const int64_t *, int64_t, hipStream_t);
 template hipError_t launch_detect<double>(const DetArgs *, int, const int64_t *, int64_t, hipStream_t);
 template hipError_t launch_copy<float>(void *, const void *, int64_t, hipStream_t);
+template hipError_t launch_detect_noalloc<float>(const void *, int64_t, double, int64_t, double, void *,
+                                                 hipStream_t);
+template hipError_t launch_detect_noalloc<double>(const void *, int64_t, double, int64_t, double, void *,
+                                                  hipStream_t);
 template hipError_t launch_copy<double>(void *, const void *, int64_t, hipStream_t);
 
 }  // namespace rs

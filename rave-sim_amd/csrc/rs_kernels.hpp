@@ -45,6 +45,20 @@ hipError_t launch_detect(const DetArgs *args, int nf, const int64_t *bounds, int
 hipError_t launch_fill_f64(double *p, int64_t n, double v, hipStream_t st);
 template <typename S> hipError_t launch_copy(void *dst, const void *src, int64_t n, hipStream_t st);
 
+// Allocation-free detector read: pixel bounds and FP64 sums of |u|^2 of one
+// natural-order field into the per-GPU scratch memory `scr` (the address
+// noalloc_scratch returned): sums at scr[0, npx) as doubles, bounds after
+// kNoAllocMaxPixels doubles.  npx <= kNoAllocMaxPixels and npx <= N.
+constexpr int64_t kNoAllocMaxPixels = 65536;
+constexpr size_t kNoAllocScratchBytes = (size_t)(2 * kNoAllocMaxPixels + 1) * 8;
+constexpr int kMaxDevices = 64;
+// device address of the scratch memory on `device`, which must be current;
+// looked up once per device (hipGetSymbolAddress)
+void *noalloc_scratch(int device);
+template <typename S>
+hipError_t launch_detect_noalloc(const void *data, int64_t N, double dx, int64_t npx, double px, void *scr,
+                                 hipStream_t st);
+
 // supported fiber lengths (log2) of the row and column passes
 constexpr int kMinRowLg = 6;
 constexpr int kMaxRowLgF32 = 13;

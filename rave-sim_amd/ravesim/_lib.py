@@ -25,13 +25,15 @@ RS_C8 = 8
 RS_C16 = 16
 RS_MODE_FUSED = 0
 RS_MODE_PLAIN = 1
+RS_NOALLOC_MAX_PIXELS = 65536
 
 # every symbol declared in include/ravesim.h
 EXPORTS = (
     "rs_version", "rs_last_error", "rs_ctx_create", "rs_ctx_destroy", "rs_ctx_synchronize",
     "rs_run_simdir", "rs_validate_simdir", "rs_field_create", "rs_field_destroy",
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
-    "rs_apply_grating", "rs_apply_dba_grating", "rs_apply_sample", "rs_scale", "rs_square_and_downsample", "rs_detector_pixels",
+    "rs_apply_grating", "rs_apply_dba_grating", "rs_apply_sample", "rs_scale", "rs_square_and_downsample", "rs_square_and_downsample_noalloc",
+    "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
     "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
     "rs_server_destroy",
@@ -144,6 +146,8 @@ def _load() -> C.CDLL:
         "rs_scale": (C.c_int, [P, C.c_double, C.c_double]),
         "rs_square_and_downsample": (C.c_int, [P, C.POINTER(rs_detector), C.c_double,
                                                C.POINTER(C.c_double), C.c_int64]),
+        "rs_square_and_downsample_noalloc": (C.c_int, [P, C.POINTER(rs_detector), C.c_double,
+                                                       C.POINTER(C.c_double), C.c_int64]),
         "rs_detector_pixels": (C.c_int64, [C.POINTER(rs_detector)]),
         "rs_detector_bounds": (C.c_int, [C.c_int64, C.c_double, C.POINTER(rs_detector),
                                          C.POINTER(C.c_int64), C.c_int64]),

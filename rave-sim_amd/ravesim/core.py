@@ -263,6 +263,21 @@ class Field:
         return out
 
 
+    def square_and_downsample_noalloc(self, size: float, px: float, py: float, z: float,
+                                      out: Optional[np.ndarray] = None) -> np.ndarray:
+        """square_and_downsample without device allocations (per-GPU scratch
+        memory, at most RS_NOALLOC_MAX_PIXELS pixels); same results."""
+        det = _lib.rs_detector(size=size, pixel_x=px, pixel_y=py)
+        n = lib.rs_detector_pixels(C.byref(det))
+        if out is None:
+            out = np.zeros(max(n, 0), dtype=np.float64)
+        if out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous float64 array")
+        check(lib.rs_square_and_downsample_noalloc(self._h, C.byref(det), z,
+                                                   out.ctypes.data_as(C.POINTER(C.c_double)), len(out)))
+        return out
+
+
 def detector_bounds(N: int, dx: float, size: float, px: float) -> np.ndarray:
     """Per-pixel sample boundaries (host computation, no device)."""
     det = _lib.rs_detector(size=size, pixel_x=px, pixel_y=1.0)
