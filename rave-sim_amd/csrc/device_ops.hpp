@@ -185,7 +185,7 @@ template <int D> constexpr int lds_step(int r) { return r * D + ((r * D) >> 4); 
 // index mask len-1: reads e, 2e, 4e, 8e and forms the other powers as one
 // product of two read values (error <= ~2 ulp, no long recurrences).
 template <typename T2, int R>
-__device__ __forceinline__ void twiddle_powers(const T2 *__restrict__ tab, int e, int mask, T2 *w) {
+__device__ __forceinline__ void twiddle_powers(const T2 *__restrict__ tab, uint32_t e, uint32_t mask, T2 *w) {
   w[0].x = 1;
   w[0].y = 0;
   if constexpr (R >= 2) w[1] = tab[e & mask];
@@ -209,8 +209,8 @@ __device__ __forceinline__ void twiddle_powers(const T2 *__restrict__ tab, int e
 // split-table twiddle W_M^e = hi[e >> sh] * lo[e & (2^sh - 1)], M = 2^lgM
 template <typename T2>
 __device__ __forceinline__ T2 split_twiddle(const T2 *__restrict__ hi, const T2 *__restrict__ lo, int sh,
-                                            int64_t e) {
-  return cmul(hi[e >> sh], lo[e & ((int64_t(1) << sh) - 1)]);
+                                            uint32_t e) {
+  return cmul(hi[e >> sh], lo[e & ((1u << sh) - 1)]);
 }
 
 template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV>
@@ -233,15 +233,18 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
       for (int r = 0; r < R; ++r) x[i][r] = src[lds_step<LR>(r)];
       // W_{NS R}^{r k} for r < R: small stages read all of them from their
       // [k][r] table in LDS; larger ones read k, 2k, 4k, 8k from W_L^j and
-      // form the other powers as products of at most two of them
+      // form the other powers as products of at most two of them (reading
+      // all fifteen from a [k][r] table in global memory instead was slower:
+      // 116 against 102 us per 2048-long row pass at 2^26, the loads' latency
+      // exposed)
       T2 w[R];
       if constexpr (NS <= kSmallNS) {
         const T2 *row = stab + small_tab_offset(L, NS) + (b % NS) * kTwRow;
 #pragma unroll
         for (int r = 1; r < R; ++r) w[r] = row[r];
       } else {
-        const int e = (b % NS) * (L / (NS * R));
-        twiddle_powers<T2, R>(twf, e, L - 1, w);
+        const uint32_t e = (uint32_t)(b % NS) * (uint32_t)(L / (NS * R));
+        twiddle_powers<T2, R>(twf, e, (uint32_t)L - 1, w);
       }
 #pragma unroll
       for (int r = 1; r < R; ++r) x[i][r] = INV ? cmulc(x[i][r], w[r]) : cmul(x[i][r], w[r]);
@@ -565,16 +568,18 @@ __device__ __forceinline__ int64_t row_kbase(const Geom &g, int64_t rr) {
 }
 
 // Inter-group twiddles of one thread's 16 elements: tw[m] = W_M^{(q + m*TPF) col},
-// M = L * S, as W_M^{q col} * (W_M^{TPF col})^m with the powers built like
-// twiddle_powers (five split-table reads instead of sixteen).
+// M = L * S, as W_M^{q col} * (W_M^{TPF col})^m with the powers built from
+// five split-table reads.  Every exponent is taken mod M (<= 2^30, a divisor
+// of 2^32), so 32-bit products that wrap give the same index.
 template <typename T2, int L>
 __device__ __forceinline__ void inter_twiddles(const Geom &g, int q, int64_t col, T2 (&tw)[16]) {
   const T2 *hi = (const T2 *)g.tw_hi;
   const T2 *lo = (const T2 *)g.tw_lo;
-  const int64_t mask = (int64_t)L * g.S - 1;
-  constexpr int TPF = L / 16;
-  const int64_t eb = ((int64_t)TPF * col) & mask;
-  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((int64_t)q * col) & mask);
+  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
+  constexpr uint32_t TPF = L / 16;
+  const uint32_t cl = (uint32_t)col;
+  const uint32_t eb = (TPF * cl) & mask;
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * cl) & mask);
   T2 b1 = split_twiddle(hi, lo, g.tw_sh, eb);
   T2 b2 = split_twiddle(hi, lo, g.tw_sh, (2 * eb) & mask);
   T2 b4 = split_twiddle(hi, lo, g.tw_sh, (4 * eb) & mask);
@@ -597,10 +602,11 @@ template <bool CONJ, typename T2, int L>
 __device__ __forceinline__ void apply_inter_twiddles(const Geom &g, int q, int64_t col, T2 (&v)[16]) {
   const T2 *hi = (const T2 *)g.tw_hi;
   const T2 *lo = (const T2 *)g.tw_lo;
-  const int64_t mask = (int64_t)L * g.S - 1;
-  constexpr int TPF = L / 16;
-  const int64_t eb = ((int64_t)TPF * col) & mask;
-  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((int64_t)q * col) & mask);
+  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
+  constexpr uint32_t TPF = L / 16;
+  const uint32_t cl = (uint32_t)col;
+  const uint32_t eb = (TPF * cl) & mask;
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * cl) & mask);
   T2 b1 = split_twiddle(hi, lo, g.tw_sh, eb);
   T2 b2 = split_twiddle(hi, lo, g.tw_sh, (2 * eb) & mask);
   T2 b4 = split_twiddle(hi, lo, g.tw_sh, (4 * eb) & mask);
@@ -616,6 +622,56 @@ __device__ __forceinline__ void apply_inter_twiddles(const Geom &g, int q, int64
     const T2 t1 = cmul(q8, p[m]);
     v[m] = CONJ ? cmulc(v[m], t0) : cmul(v[m], t0);
     v[m + 8] = CONJ ? cmulc(v[m + 8], t1) : cmul(v[m + 8], t1);
+  }
+}
+
+// Column powers of a column-pass tile, shared through LDS: the TPF threads of
+// column c need W_M^{m TPF col} (col = c0 + c, m < 16) alike, so the
+// workgroup forms each of the 16 C values once, P[m * C + c], one split-table
+// read each.  A thread's twiddles are then tw[m] = W_M^{q col} * P[m][c]:
+// one split-table read and 15 products (three roundings from the exact
+// value, against four or five for the power recurrence above).
+template <typename T2, int L, int C>
+__device__ __forceinline__ void build_col_powers(const Geom &g, int64_t c0, T2 *__restrict__ P, int t) {
+  const T2 *hi = (const T2 *)g.tw_hi;
+  const T2 *lo = (const T2 *)g.tw_lo;
+  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
+  constexpr int TPF = L / 16;
+  constexpr int NT = C * TPF;
+#pragma unroll
+  for (int i0 = 0; i0 < 16 * C; i0 += NT) {
+    const int i = i0 + t;
+    if (16 * C % NT == 0 || i < 16 * C) {
+      const uint32_t m = (uint32_t)(i / C), cl = (uint32_t)c0 + (uint32_t)(i % C);
+      P[i] = split_twiddle(hi, lo, g.tw_sh, (m * (uint32_t)TPF * cl) & mask);
+    }
+  }
+}
+
+template <typename T2, int L, int C>
+__device__ __forceinline__ void col_twiddles(const Geom &g, int q, int c, int64_t col, const T2 *__restrict__ P,
+                                             T2 (&tw)[16]) {
+  const T2 *hi = (const T2 *)g.tw_hi;
+  const T2 *lo = (const T2 *)g.tw_lo;
+  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * (uint32_t)col) & mask);
+  tw[0] = wq;
+#pragma unroll
+  for (int m = 1; m < 16; ++m) tw[m] = cmul(wq, P[m * C + c]);
+}
+
+template <bool CONJ, typename T2, int L, int C>
+__device__ __forceinline__ void apply_col_twiddles(const Geom &g, int q, int c, int64_t col,
+                                                   const T2 *__restrict__ P, T2 (&v)[16]) {
+  const T2 *hi = (const T2 *)g.tw_hi;
+  const T2 *lo = (const T2 *)g.tw_lo;
+  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
+  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * (uint32_t)col) & mask);
+  v[0] = CONJ ? cmulc(v[0], wq) : cmul(v[0], wq);
+#pragma unroll
+  for (int m = 1; m < 16; ++m) {
+    const T2 t = cmul(wq, P[m * C + c]);
+    v[m] = CONJ ? cmulc(v[m], t) : cmul(v[m], t);
   }
 }
 

@@ -239,7 +239,7 @@ static void *make_twiddles(int lgM, int64_t count, int64_t mult, bool f64) {
 // for r < 16, zero in the pad column.
 static void *make_fiber_table(int lgL, bool f64) {
   const int L = 1 << lgL;
-  const int n = L + small_tab_entries(L);
+  const int n = fiber_tab_entries(L);
   std::vector<long double> re(n, 0.0L), im(n, 0.0L);
   const long double two_pi = 6.283185307179586476925286766559005768L;
   auto w = [&](int64_t e, long double *c, long double *s) {
@@ -255,6 +255,7 @@ static void *make_fiber_table(int lgL, bool f64) {
     for (int k = 0; k < ns; ++k)
       for (int r = 0; r < 16; ++r) w((int64_t)k * r * step, &re[base + k * kTwRow + r], &im[base + k * kTwRow + r]);
   }
+
   return f64 ? upload_table<double>(re, im) : upload_table<float>(re, im);
 }
 

@@ -72,6 +72,15 @@ __global__ void __launch_bounds__(C *(L / 16))
   constexpr bool KTAB = ROW && sizeof(S) == 4;
   __shared__ float2 ktab[KTAB ? kCisTab : 1];
 
+  // column passes whose fibers span several threads share each column's
+  // twiddle powers through LDS (build_col_powers) when the extra 16 C values
+  // leave as many workgroups per CU as without them
+  constexpr int kLdsPerCu = 160 * 1024;
+  constexpr int kBase = (C * FIB + small_tab_entries(L)) * (int)sizeof(T2) + (KTAB ? kCisTab * 8 : 0);
+  constexpr int kWith = kBase + 16 * C * (int)sizeof(T2);
+  constexpr bool CPOW = !ROW && TPF >= 4 && kLdsPerCu / kWith == kLdsPerCu / kBase;
+  __shared__ T2 cpow[CPOW ? 16 * C : 1];
+
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
   const int t = threadIdx.x;
@@ -103,6 +112,11 @@ __global__ void __launch_bounds__(C *(L / 16))
   if constexpr (XV == XV_GEN) gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones);
 
   const int64_t nb = tm.nb, c0 = tm.c0;
+  // column powers first: their table reads go out before the field loads
+  if constexpr (CPOW) {
+    build_col_powers<T2, L, C>(g, c0, cpow, t);
+    __syncthreads();
+  }
   // cached grating decisions of this thread, loaded with the field (their
   // first use comes after the inverse DFT's barriers)
   uint32_t mword = 0;
@@ -128,12 +142,14 @@ __global__ void __launch_bounds__(C *(L / 16))
     bool have_tw = false;
     if (!ROW && A.do_inv) {
       if constexpr (KEEP_TW) {
-        inter_twiddles<T2, L>(g, q, col, tw);
+        if constexpr (CPOW) col_twiddles<T2, L, C>(g, q, c, col, cpow, tw);
+        else inter_twiddles<T2, L>(g, q, col, tw);
         have_tw = true;
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
       } else {
-        apply_inter_twiddles<true, T2, L>(g, q, col, v);
+        if constexpr (CPOW) apply_col_twiddles<true, T2, L, C>(g, q, c, col, cpow, v);
+        else apply_inter_twiddles<true, T2, L>(g, q, col, v);
       }
       stockham<T2, L, 1, true>(v, fib, q, twf, stab);
     }
@@ -173,6 +189,8 @@ __global__ void __launch_bounds__(C *(L / 16))
         if (have_tw) {
 #pragma unroll
           for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
+        } else if constexpr (CPOW) {
+          apply_col_twiddles<false, T2, L, C>(g, q, c, col, cpow, v);
         } else {
           apply_inter_twiddles<false, T2, L>(g, q, col, v);
         }

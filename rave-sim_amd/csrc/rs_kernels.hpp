@@ -88,6 +88,7 @@ constexpr int small_tab_offset(int L, int NS) {
   return off;
 }
 constexpr int small_tab_entries(int L) { return small_tab_offset(L, L); }
+constexpr int fiber_tab_entries(int L) { return L + small_tab_entries(L); }
 
 // fibers per workgroup of a column pass of length L: row segments of 256 B
 // (512 B for complex64 fibers up to 64) and at least one wavefront up to
