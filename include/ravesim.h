@@ -51,7 +51,7 @@ typedef struct rs_field rs_field;
 /* Library version (major*10000 + minor*100 + patch).  The minor number
  * changes with the layout of any struct below; a binding checks it before
  * passing structs. */
-#define RS_VERSION 200
+#define RS_VERSION 300
 int rs_version(void);
 
 /* Message of this thread's last failed call ("" if none). */
@@ -359,6 +359,30 @@ int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, d
  * top column length, -1 = default); persisted tuning results are re-applied
  * this way. */
 int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA);
+
+/* One autotuner result: the digit-group split and the column tile variants
+ * (0 = the default width, 1 = the alternative: twice as many columns per
+ * workgroup for 128- and 512-long columns, half as many otherwise) chosen
+ * for (N, dtype) at a batch size of `members` fields (0: every batch size). */
+typedef struct rs_tune_entry {
+  int64_t N;
+  int32_t dtype;
+  int32_t members;
+  int32_t lgR, lgA;
+  int32_t tileA, tileB;  /* top and middle column passes */
+  double ms_per_slice;
+} rs_tune_entry;
+
+/* The entry runs of `members` fields would use (the nearest tuned batch size
+ * in log2, or a fixed split); *found = 0 when there is none. */
+int rs_tune_get(rs_ctx *ctx, int64_t N, int dtype, int members, rs_tune_entry *out, int32_t *found);
+/* Persist the context's tuning entries as text (one line per entry:
+ * device lgN c8|c16 members lgR lgA tileA tileB ms_per_slice, where device
+ * is the architecture name and compute-unit count), or read such a file:
+ * entries of this context's device are added (same key replaced), *n_loaded
+ * receives their count.  A malformed or invalid entry is RS_ERR_CONFIG. */
+int rs_tune_save(rs_ctx *ctx, const char *path);
+int rs_tune_load(rs_ctx *ctx, const char *path, int32_t *n_loaded);
 
 /* Per-launch event timing: 0 = off, 1 = every launch, k > 1 = every k-th
  * launch of each kernel class (a sample; each timed launch costs a few

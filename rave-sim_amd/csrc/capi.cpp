@@ -560,6 +560,34 @@ int rs_set_split(rs_ctx *ctx, int64_t N, int dtype, int lgR, int lgA) {
   });
 }
 
+int rs_tune_get(rs_ctx *ctx, int64_t N, int dtype, int members, rs_tune_entry *out, int32_t *found) {
+  if (!ctx || !out || !found) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
+  if (dtype != RS_C8 && dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  return guarded([&] {
+    int lgN = 0;
+    while ((int64_t(1) << lgN) < N) ++lgN;
+    const rs::Context::Tuned *t = ctx->ctx->find_tuned(lgN, dtype == RS_C16, members);
+    *found = t ? 1 : 0;
+    if (t) *out = rs_tune_entry{N, dtype, t->members, t->lgR, t->lgA, t->tileA, t->tileB, t->ms};
+  });
+}
+
+int rs_tune_save(rs_ctx *ctx, const char *path) {
+  if (!ctx || !path) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
+  return guarded([&] { rs::tune_save(*ctx->ctx, path); });
+}
+
+int rs_tune_load(rs_ctx *ctx, const char *path, int32_t *n_loaded) {
+  if (!ctx || !path) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
+  return guarded([&] {
+    const int n = rs::tune_load(*ctx->ctx, path);
+    if (n_loaded) *n_loaded = n;
+  });
+}
+
 int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out) {
   if (!ctx || !out) return fail(RS_ERR_INVALID, "NULL argument");
   if (reserved(ctx)) return fail_reserved();

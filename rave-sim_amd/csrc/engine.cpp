@@ -259,7 +259,17 @@ static void *make_fiber_table(int lgL, bool f64) {
   return f64 ? upload_table<double>(re, im) : upload_table<float>(re, im);
 }
 
-std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
+bool plan_tiles_valid(int lgN, bool f64, int lgR, int lgA, int tileA, int tileB) {
+  int ng, a, b, r;
+  plan_split(lgN, f64, lgR, lgA, &ng, &a, &b, &r);
+  if (ng == 1) return tileA == 0 && tileB == 0;
+  if (ng == 2 && tileB != 0) return false;
+  if ((int64_t(1) << (b + r)) % col_fibers(1 << a, f64, tileA) != 0) return false;
+  if (ng == 3 && (int64_t(1) << r) % col_fibers(1 << b, f64, tileB) != 0) return false;
+  return true;
+}
+
+std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req, int tileA, int tileB) {
   int lgN = 0;
   while ((int64_t(1) << lgN) < N) ++lgN;
   if ((int64_t(1) << lgN) != N) throw Error(6, "N must be a power of two");
@@ -268,6 +278,12 @@ std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req, int lgA_req) {
   p->lgN = lgN;
   p->f64 = f64;
   plan_split(lgN, f64, lgR_req, lgA_req, &p->ngroups, &p->lgA, &p->lgB, &p->lgR);
+  if (tileA < 0 || tileA >= kColTiles || tileB < 0 || tileB >= kColTiles) throw Error(1, "tile variant out of range");
+  if (!plan_tiles_valid(lgN, f64, p->lgR, p->ngroups >= 2 ? p->lgA : p->lgR, p->ngroups >= 2 ? tileA : 0,
+                        p->ngroups == 3 ? tileB : 0))
+    throw Error(6, "column tile does not fit this split");
+  p->tileA = p->ngroups >= 2 ? tileA : 0;
+  p->tileB = p->ngroups == 3 ? tileB : 0;
   // in-fiber tables: row, top, middle
   p->fib[0] = make_fiber_table(p->lgR, f64);
   if (p->ngroups >= 2) p->fib[1] = make_fiber_table(p->lgA, f64);
@@ -307,15 +323,16 @@ Geom Plan::geom_row() const {
   g.cis_tab = cis;
   return g;
 }
-static int32_t line_sharing(int lgL, bool f64) {
-  const int bytes = col_fibers(1 << lgL, f64) * (f64 ? 16 : 8);
+static int32_t line_sharing(int lgL, bool f64, int tile) {
+  const int bytes = col_fibers(1 << lgL, f64, tile) * (f64 ? 16 : 8);
   return bytes >= 128 ? 1 : 128 / bytes;
 }
 
 Geom Plan::geom_top() const {
   Geom g = geom_row();
   g.top = 1;
-  g.xcd_segs = line_sharing(lgA, f64);
+  g.xcd_segs = line_sharing(lgA, f64, tileA);
+  g.tile = tileA;
   g.S = int64_t(1) << (lgB + lgR);
   g.tw_fib = fib[1];
   g.tw_sh = sh[0];
@@ -326,7 +343,8 @@ Geom Plan::geom_top() const {
 Geom Plan::geom_mid() const {
   Geom g = geom_row();
   g.top = 0;
-  g.xcd_segs = line_sharing(lgB, f64);
+  g.xcd_segs = line_sharing(lgB, f64, tileB);
+  g.tile = tileB;
   g.S = int64_t(1) << lgR;
   g.tw_fib = fib[2];
   g.tw_sh = sh[1];
@@ -340,6 +358,7 @@ std::string Plan::describe() const {
   if (ngroups == 1) s << " [row 2^" << lgR << "]";
   else if (ngroups == 2) s << " [col 2^" << lgA << " | row 2^" << lgR << "]";
   else s << " [col 2^" << lgA << " | col 2^" << lgB << " | row 2^" << lgR << "]";
+  if (tileA || tileB) s << " tiles " << tileA << tileB;
   return s.str();
 }
 
@@ -347,6 +366,11 @@ Context::Context(int dev) : device(dev) {
   hip_check(hipSetDevice(dev), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   noalloc_scr = noalloc_scratch(dev);
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  device_key = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+  for (char &ch : device_key)
+    if (ch == ' ') ch = '_';
 }
 Context::~Context() {
   plans_.clear();
@@ -546,20 +570,24 @@ const Context::Tuned *Context::find_tuned(int lgN, bool f64, int members) const 
   return best;
 }
 
-Plan &Context::plan(int64_t N, bool f64, int lgR, int lgA, int members) {
+Plan &Context::plan(int64_t N, bool f64, int lgR, int lgA, int members, int tileA, int tileB) {
   if (lgR < 0 && lgA < 0) {
     int lgN = 0;
     while ((int64_t(1) << lgN) < N) ++lgN;
     if (const Tuned *t = find_tuned(lgN, f64, members)) {
       lgR = t->lgR;
       lgA = t->lgA;
+      if (tileA < 0) tileA = t->tileA;
+      if (tileB < 0) tileB = t->tileB;
     }
   }
+  tileA = std::max(tileA, 0);
+  tileB = std::max(tileB, 0);
+  const std::array<int, 4> req{lgR, lgA, tileA, tileB};
   for (size_t i = 0; i < plans_.size(); ++i)
-    if (plans_[i]->N == N && plans_[i]->f64 == f64 && plan_req_[i] == std::make_pair(lgR, lgA))
-      return *plans_[i];
-  plans_.push_back(make_plan(N, f64, lgR, lgA));
-  plan_req_.emplace_back(lgR, lgA);
+    if (plans_[i]->N == N && plans_[i]->f64 == f64 && plan_req_[i] == req) return *plans_[i];
+  plans_.push_back(make_plan(N, f64, lgR, lgA, tileA, tileB));
+  plan_req_.push_back(req);
   return *plans_.back();
 }
 void Context::sync() {
@@ -623,7 +651,8 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
   }
   lanes = std::max(1, std::min({lanes, 4, (int)members.size()}));
   if (lanes > 1 && rc.max_resident <= 0) group = std::max(1, group / lanes);
-  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535));
+  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535), rc.tileA,
+                        rc.tileB);
   std::vector<hipStream_t> lane_st(lanes);
   for (int l = 0; l < lanes; ++l) lane_st[l] = ctx.lane_stream(l);
 
@@ -854,22 +883,28 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members
   const bool was = ctx.timing;
   ctx.timing = false;
   Context::Tuned best{lgN, f64, -1, -1, 1e300, key};
-  for (auto cand : plan_candidates(lgN, f64)) {
-    rc.lgR = cand.first;
-    rc.lgA = cand.second;
-    std::vector<Member> ms(run_n, m);
-    (void)run_members(ctx, rc, ms);  // warm: plan tables, code objects
-    double t_best = 1e300;
-    for (int rep = 0; rep < 2; ++rep) {
-      ms.assign(run_n, m);
-      hip_check(hipStreamSynchronize(ctx.stream), "sync");
-      const auto t0 = std::chrono::steady_clock::now();
-      (void)run_members(ctx, rc, ms);
-      t_best = std::min(t_best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-    }
-    const double per = t_best / (reps + 1) / run_n;
-    if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, key};
-  }
+  for (auto cand : plan_candidates(lgN, f64))
+    for (int ta = 0; ta < kColTiles; ++ta)
+      for (int tb = 0; tb < kColTiles; ++tb) {
+        if (!plan_tiles_valid(lgN, f64, cand.first, cand.second, ta, tb)) continue;
+        rc.lgR = cand.first;
+        rc.lgA = cand.second;
+        rc.tileA = ta;
+        rc.tileB = tb;
+        std::vector<Member> ms(run_n, m);
+        (void)run_members(ctx, rc, ms);  // warm: plan tables, code objects
+        double t_best = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+          ms.assign(run_n, m);
+          hip_check(hipStreamSynchronize(ctx.stream), "sync");
+          const auto t0 = std::chrono::steady_clock::now();
+          (void)run_members(ctx, rc, ms);
+          t_best =
+              std::min(t_best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        const double per = t_best / (reps + 1) / run_n;
+        if (per < best.ms) best = Context::Tuned{lgN, f64, cand.first, cand.second, per, key, ta, tb};
+      }
   ctx.timing = was;
   bool replaced = false;
   for (auto &t : ctx.tuned)
@@ -880,6 +915,58 @@ Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members
     }
   if (!replaced) ctx.tuned.push_back(best);
   return best;
+}
+
+void tune_save(const Context &ctx, const std::string &path) {
+  std::ostringstream o;
+  o << "# ravesim tuning table v1: device lgN dtype members lgR lgA tileA tileB ms_per_slice\n";
+  o.precision(6);
+  for (const auto &t : ctx.tuned)
+    o << ctx.device_key << ' ' << t.lgN << ' ' << (t.f64 ? "c16" : "c8") << ' ' << t.members << ' ' << t.lgR << ' '
+      << t.lgA << ' ' << t.tileA << ' ' << t.tileB << ' ' << t.ms << '\n';
+  const std::string txt = o.str();
+  FILE *f = std::fopen(path.c_str(), "w");
+  if (!f) throw Error(3, "cannot write " + path);
+  const size_t n = std::fwrite(txt.data(), 1, txt.size(), f);
+  const bool ok = n == txt.size() && std::fclose(f) == 0;
+  if (!ok) throw Error(3, "write failed: " + path);
+}
+
+int tune_load(Context &ctx, const std::string &path) {
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) throw Error(3, "cannot read " + path);
+  std::string txt;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, n);
+  std::fclose(f);
+  std::istringstream in(txt);
+  std::string line;
+  int taken = 0, lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    if (line.empty() || line[0] == '#') continue;
+    std::istringstream ls(line);
+    std::string dev, dt;
+    Context::Tuned t{};
+    if (!(ls >> dev >> t.lgN >> dt >> t.members >> t.lgR >> t.lgA >> t.tileA >> t.tileB >> t.ms) ||
+        (dt != "c8" && dt != "c16"))
+      throw Error(2, path + ":" + std::to_string(lineno) + ": malformed tuning entry");
+    if (dev != ctx.device_key) continue;
+    t.f64 = dt == "c16";
+    if (t.lgN < kMinRowLg || t.lgN > 30 || t.members < 0 || t.tileA < 0 || t.tileA >= kColTiles || t.tileB < 0 ||
+        t.tileB >= kColTiles || !plan_tiles_valid(t.lgN, t.f64, t.lgR, t.lgA, t.tileA, t.tileB))
+      throw Error(2, path + ":" + std::to_string(lineno) + ": invalid split or tiles");
+    auto &tv = ctx.tuned;
+    tv.erase(std::remove_if(tv.begin(), tv.end(),
+                            [&](const Context::Tuned &u) {
+                              return u.lgN == t.lgN && u.f64 == t.f64 && u.members == t.members;
+                            }),
+             tv.end());
+    tv.push_back(t);
+    ++taken;
+  }
+  return taken;
 }
 
 }  // namespace rs

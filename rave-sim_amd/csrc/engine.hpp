@@ -10,6 +10,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <functional>
@@ -73,6 +74,7 @@ struct Plan {
   bool f64 = false;
   int ngroups = 0;
   int lgA = 0, lgB = 0, lgR = 0;  // top, middle, row (lgA = lgR when ngroups == 1)
+  int tileA = 0, tileB = 0;       // column tile variants of the top and middle passes (col_fibers)
   // device tables (owned)
   void *fib[3] = {nullptr, nullptr, nullptr};  // per pass: W_L^j, then small-stage tables
   void *hi[2] = {nullptr, nullptr}, *lo[2] = {nullptr, nullptr};
@@ -91,7 +93,11 @@ struct Plan {
 // split of lgN into digit groups; lgR_req / lgA_req >= 0 override the default
 // (row as long as the kernels allow, remaining bits halved between columns)
 void plan_split(int lgN, bool f64, int lgR_req, int lgA_req, int *ng, int *lgA, int *lgB, int *lgR);
-std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req = -1, int lgA_req = -1);
+std::unique_ptr<Plan> make_plan(int64_t N, bool f64, int lgR_req = -1, int lgA_req = -1, int tileA = 0,
+                                int tileB = 0);
+// whether the column tiles (tileA, tileB) fit the split lgR | lgA of lgN (the
+// tile width divides the pass's stride)
+bool plan_tiles_valid(int lgN, bool f64, int lgR, int lgA, int tileA, int tileB);
 // every valid (lgR, lgA) split of lgN, the autotuner's candidates
 std::vector<std::pair<int, int>> plan_candidates(int lgN, bool f64);
 
@@ -154,6 +160,7 @@ struct RunConfig {
   int spans = 0;             // sub-ranges of the middle-row-middle passes (3 groups); 0 = by cache size
   int lgR = -1;              // row length override (autotuner), -1 = tuned or default
   int lgA = -1;              // top column length override
+  int tileA = -1, tileB = -1;  // column tile variants with an override (-1 = default)
   void *const *fields = nullptr;  // caller-owned field buffers (one per member), else allocated
 };
 
@@ -190,9 +197,10 @@ class Context {
   int timing_stride = 1;              // time every timing_stride-th launch of a class
   int64_t timing_seen[PC_COUNT] = {};  // launches of each class since timing was set
   PassStat stats[PC_COUNT];
-  // plan for (N, precision); explicit lgR/lgA, else the split tuned for the
-  // nearest batch size (fields per run), else the default split
-  Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1, int members = 1);
+  // plan for (N, precision); explicit lgR/lgA (with tiles tileA/tileB, -1 =
+  // default), else the split and tiles tuned for the nearest batch size
+  // (fields per run), else the default split
+  Plan &plan(int64_t N, bool f64, int lgR = -1, int lgA = -1, int members = 1, int tileA = -1, int tileB = -1);
   void sync();
   // autotuner results: (lgN, f64, fields per run) -> (lgR, lgA, ms per slice
   // and field); members == 0 marks a fixed split for every batch size
@@ -202,7 +210,11 @@ class Context {
     int lgR, lgA;
     double ms;
     int members;
+    int tileA = 0, tileB = 0;
   };
+  // identity of the device for persisted tuning tables: architecture name
+  // and compute-unit count (e.g. "gfx950:sramecc+:xnack-/256")
+  std::string device_key;
   std::vector<Tuned> tuned;
   const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
   // Copy N natural-order samples of a .npy vector (member m's upload_npy)
@@ -250,14 +262,23 @@ class Context {
                  const std::function<void(char *, int64_t, int64_t)> &fill);
   std::vector<hipStream_t> lanes_;
   std::vector<std::unique_ptr<Plan>> plans_;
-  std::vector<std::pair<int, int>> plan_req_;
+  std::vector<std::array<int, 4>> plan_req_;  // lgR, lgA, tileA, tileB as requested
 };
 
-// Time every candidate split of N on a synthetic grating slice loop (mask +
-// propagation) run as a batch of `members` fields runs by default (grouping
-// and lanes of run_members), by wall clock, and record the fastest in
-// ctx.tuned for that batch size.  Returns it.
+// Time every candidate split of N, with each column tile variant of its
+// column passes, on a synthetic grating slice loop (mask + propagation) run
+// as a batch of `members` fields runs by default (grouping and lanes of
+// run_members), by wall clock, and record the fastest in ctx.tuned for that
+// batch size.  Returns it.
 Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3, int members = 1);
+
+// Tuning tables as text, one entry per line:
+//   device lgN c8|c16 members lgR lgA tileA tileB ms_per_slice
+// (members 0: a fixed split for every batch size).  save writes every entry
+// of the context; load adds the entries recorded for this context's device
+// (replacing entries with the same key) and returns how many it took.
+void tune_save(const Context &ctx, const std::string &path);
+int tune_load(Context &ctx, const std::string &path);
 
 // device buffer RAII
 struct DevBuf {

@@ -245,14 +245,19 @@ __global__ void __launch_bounds__(C *(L / 16)) mask_fill_kernel(XOp x, Geom g, u
   bits[(int64_t)blockIdx.x * (C * (L / 16)) + threadIdx.x] = (uint16_t)w;
 }
 
-template <typename S, int L>
-static hipError_t launch_mask_fill_l(const XOp &x, const Geom &g, void *bits, hipStream_t st) {
-  constexpr int C = col_fibers(L, sizeof(S) == 8);
+template <typename S, int L, int TILE>
+static hipError_t launch_mask_fill_c(const XOp &x, const Geom &g, void *bits, hipStream_t st) {
+  constexpr int C = col_fibers(L, sizeof(S) == 8, TILE);
   if (g.S % C != 0) return hipErrorInvalidValue;
   const int64_t nblk = g.N / ((int64_t)L * C);
   hipLaunchKernelGGL((mask_fill_kernel<L, C>), dim3((unsigned)nblk), dim3(C * (L / 16)), 0, st, x, g,
                      (uint16_t *)bits);
   return hipGetLastError();
+}
+
+template <typename S, int L>
+static hipError_t launch_mask_fill_l(const XOp &x, const Geom &g, void *bits, hipStream_t st) {
+  return g.tile ? launch_mask_fill_c<S, L, 1>(x, g, bits, st) : launch_mask_fill_c<S, L, 0>(x, g, bits, st);
 }
 
 template <typename S>
@@ -289,12 +294,12 @@ static inline bool span_tiles(const Geom &g, int64_t per, Geom &gs, int64_t &nbl
   return true;
 }
 
-// columns per workgroup for a column pass of fiber length L: 16 columns of
-// complex64 (128 B per row segment) / 8 of complex128, more for short fibers
-template <typename S, int L, int XV>
-static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
-                             hipEvent_t t1) {
-  constexpr int C = col_fibers(L, sizeof(S) == 8);
+// columns per workgroup for a column pass of fiber length L: col_fibers of
+// the plan's tile variant (g.tile)
+template <typename S, int L, int XV, int TILE>
+static hipError_t launch_col_c(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                               hipEvent_t t1) {
+  constexpr int C = col_fibers(L, sizeof(S) == 8, TILE);
   if (g.S % C != 0) return hipErrorInvalidValue;
   Geom gs;
   int64_t nblk;
@@ -302,6 +307,12 @@ static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipSt
   dim3 grid((unsigned)nblk, (unsigned)nf);
   hipExtLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0, args, gs);
   return hipGetLastError();
+}
+
+template <typename S, int L, int XV>
+static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                             hipEvent_t t1) {
+  return g.tile ? launch_col_c<S, L, XV, 1>(args, nf, g, st, t0, t1) : launch_col_c<S, L, XV, 0>(args, nf, g, st, t0, t1);
 }
 
 // rows per workgroup: full waves and 256 threads (complex64) / 128 threads

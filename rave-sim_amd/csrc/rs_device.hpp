@@ -109,6 +109,8 @@ struct Geom {
   // inter-group twiddle W_M^e, e = k*lo, M = L*S, split as hi[e >> sh] * lo[e & mask]
   int32_t tw_sh;
   int32_t xcd_segs;  // column passes: workgroups sharing one 128 B line (1 = none)
+  int32_t tile;      // column passes: tile width variant (col_fibers), 0 = default
+  int32_t pad_;
   const void *tw_hi;
   const void *tw_lo;
   const void *tw_fib;  // W_L^j, j < L, then the small-stage tables (rs_kernels.hpp)

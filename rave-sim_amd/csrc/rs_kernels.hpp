@@ -90,16 +90,24 @@ constexpr int small_tab_offset(int L, int NS) {
 constexpr int small_tab_entries(int L) { return small_tab_offset(L, L); }
 constexpr int fiber_tab_entries(int L) { return L + small_tab_entries(L); }
 
-// fibers per workgroup of a column pass of length L: row segments of 256 B
-// (512 B for complex64 fibers up to 64) and at least one wavefront up to
-// L = 256; 128 B at L = 512, where 256 B would leave one workgroup per CU;
-// beyond that C * L fixed at 128 KB of LDS
-constexpr int col_fibers(int L, bool f64) {
-  if (L >= 1024) return (f64 ? 8192 : 16384) / L;
-  if (L == 512) return f64 ? 8 : 16;
-  if (L <= 16) return 64;
-  if (f64) return L <= 32 ? 32 : 16;
-  return L <= 64 ? 64 : 32;
+// fibers per workgroup of a column pass of length L (tile 0, the default):
+// row segments of 256 B (512 B for complex64 fibers up to 64) and at least
+// one wavefront up to L = 256; 128 B at L = 512, where 256 B would leave one
+// workgroup per CU; beyond that C * L fixed at 128 KB of LDS.  Tile 1 is the
+// alternative the autotuner may pick per (N, precision, batch size): twice
+// as many fibers at L = 128 and 512 (and complex128 at 64), half as many
+// elsewhere (64 KB wide tiles, two workgroups per CU).
+constexpr int col_fibers(int L, bool f64, int tile = 0) {
+  int c;
+  if (L >= 1024) c = (f64 ? 8192 : 16384) / L;
+  else if (L == 512) c = f64 ? 8 : 16;
+  else if (L <= 16) c = 64;
+  else if (f64) c = L <= 32 ? 32 : 16;
+  else c = L <= 64 ? 64 : 32;
+  if (tile == 0) return c;
+  const bool wider = L == 128 || L == 512 || (f64 && L == 64);
+  return wider ? 2 * c : (c > 1 ? c / 2 : c);
 }
+constexpr int kColTiles = 2;
 
 }  // namespace rs

@@ -35,6 +35,7 @@ EXPORTS = (
     "rs_apply_grating", "rs_apply_dba_grating", "rs_apply_sample", "rs_scale", "rs_square_and_downsample", "rs_square_and_downsample_noalloc",
     "rs_detector_pixels",
     "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
+    "rs_tune_get", "rs_tune_save", "rs_tune_load",
     "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
     "rs_server_destroy",
 )
@@ -111,12 +112,17 @@ class rs_request(C.Structure):
                 ("source_z", C.c_double)]
 
 
+class rs_tune_entry(C.Structure):
+    _fields_ = [("N", C.c_int64), ("dtype", C.c_int32), ("members", C.c_int32), ("lgR", C.c_int32),
+                ("lgA", C.c_int32), ("tileA", C.c_int32), ("tileB", C.c_int32), ("ms_per_slice", C.c_double)]
+
+
 class rs_pass_stat(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
 # include/ravesim.h RS_VERSION: the struct layouts declared above
-VERSION = 200
+VERSION = 300
 
 
 def _load() -> C.CDLL:
@@ -155,6 +161,9 @@ def _load() -> C.CDLL:
         "rs_set_timing": (C.c_int, [P, C.c_int]),
         "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "rs_set_split": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.c_int]),
+        "rs_tune_get": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.POINTER(rs_tune_entry), C.POINTER(C.c_int32)]),
+        "rs_tune_save": (C.c_int, [P, C.c_char_p]),
+        "rs_tune_load": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_int32)]),
         "rs_server_create": (C.c_int, [P, C.c_double, C.c_int, C.POINTER(P)]),
         "rs_server_submit": (C.c_int, [P, C.POINTER(rs_request), C.POINTER(C.c_int64)]),
         "rs_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int64,

@@ -52,7 +52,29 @@ class Context:
         ms = C.c_double()
         code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16
         check(lib.rs_autotune(self.handle, int(N), code, int(members), out, C.byref(ms)))
-        return {"groups": out[0], "lgA": out[1], "lgB": out[2], "lgR": out[3], "ms_per_slice": ms.value}
+        t = self.tuned(N, dtype, members) or {}
+        return {"groups": out[0], "lgA": out[1], "lgB": out[2], "lgR": out[3], "tileA": t.get("tileA", 0),
+                "tileB": t.get("tileB", 0), "ms_per_slice": ms.value}
+
+    def tuned(self, N: int, dtype=np.complex64, members: int = 1) -> Optional[dict]:
+        """The tuning entry runs of `members` fields use, or None."""
+        e = _lib.rs_tune_entry()
+        found = C.c_int32()
+        code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16
+        check(lib.rs_tune_get(self.handle, int(N), code, int(members), C.byref(e), C.byref(found)))
+        if not found.value:
+            return None
+        return {"lgR": e.lgR, "lgA": e.lgA, "tileA": e.tileA, "tileB": e.tileB, "members": e.members,
+                "ms_per_slice": e.ms_per_slice}
+
+    def tune_save(self, path) -> None:
+        check(lib.rs_tune_save(self.handle, str(path).encode()))
+
+    def tune_load(self, path) -> int:
+        """Add the entries of a tuning table recorded for this device; returns their count."""
+        n = C.c_int32()
+        check(lib.rs_tune_load(self.handle, str(path).encode(), C.byref(n)))
+        return n.value
 
     def set_split(self, N: int, dtype, lgR: int, lgA: int = -1) -> None:
         code = _lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16

@@ -485,6 +485,80 @@ def test_every_column_tile(ctx, dtype, lgR, lgA, lgB):
     assert rel_l2(got, ref.u) < tol
 
 
+def set_plan_tiles(c, path, N, dtype, lgR, lgA, tileA, tileB):
+    """Fix split and column tiles for every batch size through a loaded tuning table."""
+    c.set_split(N, dtype, lgR, lgA)
+    c.tune_save(path)
+    dev = next(l for l in path.read_text().splitlines() if not l.startswith("#")).split()[0]
+    dt = "c8" if np.dtype(dtype) == np.complex64 else "c16"
+    path.write_text(f"{dev} {int(math.log2(N))} {dt} 0 {lgR} {lgA} {tileA} {tileB} 0\n")
+    assert c.tune_load(path) == 1
+    t = c.tuned(N, dtype, 1)
+    assert (t["lgR"], t["lgA"], t["tileA"], t["tileB"]) == (lgR, lgA, tileA, tileB)
+
+
+@pytest.mark.parametrize("dtype,lgR,lgA,lgB", COL_CASES)
+def test_every_column_tile_alternative(ctx, tmp_path, dtype, lgR, lgA, lgB):
+    """The autotuner's alternative tile width of every column length, top
+    (grating mask inside) and middle, against the oracle."""
+    import ravesim
+    N, dx = 2 ** (lgR + lgA + lgB), 2e-9
+    set_plan_tiles(ctx, tmp_path / "tiles.txt", N, dtype, lgR, lgA, 1 if lgB == 0 else 0, 1 if lgB else 0)
+    wl = ravesim.energy_to_wavelength(30000.0)
+    gk = dict(pitch=0.37e-6, dc=(0.4, 0.55), z_start=0.0, thickness=6e-6, nr_steps=3, db_a=complex(2e-6, 4e-8))
+    u = rand_field(N, 11 * lgA + lgB)
+    fc = 0.4 / dx
+    f = ravesim.Field(ctx, N, dx, dtype=dtype)
+    f.upload(u)
+    f.apply_grating(wl, ravesim.Grating(x_positions=[3.1e-8], **gk), cutoff_freq=fc)
+    f.propagate(wl, 1.3e-3, fc)
+    got = f.download()
+    f.close()
+    p = bw.SimParams(N=N, dx=dx, z_detector=1.0, detector_size=1e-6, detector_pixel_size_x=1e-7,
+                     detector_pixel_size_y=1.0, wl=wl, chunk_size=N)
+    ref = bw.Fields(u)
+    bw.grating_apply(ref, bw.Grating(x_positions=np.array([3.1e-8]), substrate_thickness=0.0, **gk), p, fc, 0)
+    bw.propagate(ref, dx, wl, 1.3e-3, N, fc)
+    tol = 1e-10 if dtype == np.complex128 else 2e-6
+    assert rel_l2(got, ref.u) < tol
+
+
+def test_tuning_table_persists_across_contexts(tmp_path):
+    """A table saved by one context and loaded by another gives the same
+    plan (split and tiles) and bit-identical rows; a malformed table is a
+    configuration error and entries of another device are skipped."""
+    import ravesim
+    from ravesim._lib import RS_ERR_CONFIG, RsError
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 18, 2e-9
+    ms = [BatchMember(wl=ravesim.energy_to_wavelength((30 + 7 * k) * 1e3),
+                      gratings=[ravesim.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=30e-6,
+                                                nr_steps=3, db_a=complex(7.7e-7, 3.5e-10))],
+                      cutoff_angles=[math.pi / 2] * 2, plane_wave=True) for k in range(4)]
+    kw = dict(det_size=2e-4, det_px=1e-6, det_py=1.0, z_detector=0.05)
+    a = ravesim.Context(0)
+    ta = a.autotune(N, np.complex64, members=4)
+    path = tmp_path / "tuned.txt"
+    a.tune_save(path)
+    rows_a = run_batch(a, N, dx, ms, **kw)
+    b = ravesim.Context(0)
+    assert b.tuned(N, np.complex64, 4) is None
+    assert b.tune_load(path) >= 1
+    tb = b.tuned(N, np.complex64, 4)
+    assert (tb["lgR"], tb["lgA"], tb["tileA"], tb["tileB"]) == (ta["lgR"], ta["lgA"], ta["tileA"], ta["tileB"])
+    assert np.array_equal(run_batch(b, N, dx, ms, **kw), rows_a)
+    other = tmp_path / "other.txt"
+    other.write_text("gfx000:other/1 18 c8 4 9 9 0 0 0.1\n")
+    assert b.tune_load(other) == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text(path.read_text().splitlines()[1].rsplit(" ", 3)[0] + "\n")
+    with pytest.raises(RsError) as e:
+        b.tune_load(bad)
+    assert e.value.code == RS_ERR_CONFIG
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("dtype,tol", [("c16", 1e-10), ("c8", 1e-4)])
 def test_sample_element_matches_oracle(ctx, tmp_path, dtype, tol):
     """Sample grid rows (np.interp of delta + i beta between pixel centres,
