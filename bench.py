@@ -21,6 +21,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "rave-sim_amd"))
@@ -65,6 +67,24 @@ def poly_members(ravesim, N, n_energies=64, slices=32):
     return ms, h + 0.1, slices + 1
 
 
+TUNE_TABLE = ROOT / "rave-sim_amd" / "ravesim" / "data" / "tuned_mi355x.txt"
+TUNE_LOG = []
+
+
+def ensure_tuned(ctx, N, dtype, members=1, retune=False):
+    """The tuning entry for exactly (N, dtype, members): taken from the loaded
+    table when it has one, else measured by the autotuner now (its seconds
+    are logged in TUNE_LOG)."""
+    t = None if retune else ctx.tuned(N, dtype, members)
+    if t is not None and t["members"] == members:
+        return {**t, "tuned_now_s": 0.0}
+    t0 = time.perf_counter()
+    ctx.autotune(N, dtype, members=members)
+    dt = time.perf_counter() - t0
+    TUNE_LOG.append({"N": N, "dtype": np.dtype(dtype).name, "members": members, "s": round(dt, 3)})
+    return {**ctx.tuned(N, dtype, members), "tuned_now_s": round(dt, 3)}
+
+
 def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
     """Batched fused complex64 form vs the plain complex128 form run one
     simulation at a time, on the polychromatic batch (sims/hour each)."""
@@ -73,7 +93,7 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
     N = 2 ** lgN
     ms, zdet, steps = poly_members(ravesim, N)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet)
-    tuned = ctx.autotune(N, np.complex64, members=len(ms))
+    tuned = ensure_tuned(ctx, N, np.complex64, len(ms))
     run_batch(ctx, N, 2e-9, ms[:2], dtype=np.complex64, **kw)
     rep_s = []
     for _ in range(reps):
@@ -85,7 +105,7 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
     plain_runs = {}
     for label in ("default", "tuned"):
         if label == "tuned":
-            plain_runs["tuned_split"] = ctx.autotune(N, np.complex128, members=1)
+            plain_runs["tuned_split"] = ensure_tuned(ctx, N, np.complex128, 1)
         run_batch(ctx, N, 2e-9, ms[:1], dtype=np.complex128, plain=True, max_resident=1, **kw)
         t0 = time.perf_counter()
         plain = run_batch(ctx, N, 2e-9, ms, dtype=np.complex128, plain=True, max_resident=1, **kw)
@@ -128,12 +148,14 @@ def uneven_mix_sims(ravesim, n_sims=256, seed=11, slices=32):
     return sims, h + 0.1, slices + 1
 
 
-def uneven_mix(ravesim, ctx, n_sims=256):
-    """Fused complex64 batches (one lockstep batch per grid size, split tuned
-    per size and batch size, each simulation's sources summed into its own
-    detector row) vs the plain complex128 form one simulation at a time."""
+def uneven_mix(ravesim, ctx, n_sims=256, retune=False):
+    """All simulations in one mixed call (rs_batch_run_mixed: one lockstep
+    part per grid size, split and tiles tuned per size and batch size, each
+    simulation's sources summed into its own detector row), eager and as one
+    captured hipGraph over both lanes, vs the plain complex128 form one
+    simulation at a time."""
     import numpy as np
-    from ravesim.core import run_batch
+    from ravesim.core import BatchSim, run_batch, run_mixed
     sims, zdet, steps = uneven_mix_sims(ravesim, n_sims)
     dx = 2e-9
     by_n = {}
@@ -143,27 +165,22 @@ def uneven_mix(ravesim, ctx, n_sims=256):
     def kw(lgN):
         return dict(det_size=(2 ** lgN) * dx / 4, det_px=1e-6, det_py=1.0, z_detector=zdet)
 
-    def fused_run():
-        out = {}
-        for lgN, idx in sorted(by_n.items()):
-            ms = []
-            for r, i in enumerate(idx):
-                for m in sims[i][1]:
-                    m.out_index = r
-                    ms.append(m)
-            rows = run_batch(ctx, 2 ** lgN, dx, ms, n_outputs=len(idx), dtype=np.complex64, **kw(lgN))
-            for r, i in enumerate(idx):
-                out[i] = rows[r]
-        return out
+    mixed = [BatchSim(N=2 ** lgN, dx=dx, members=srcs, **kw(lgN)) for lgN, srcs in sims]
 
     t0 = time.perf_counter()
     for lgN, idx in sorted(by_n.items()):
-        ctx.autotune(2 ** lgN, np.complex64, members=sum(len(sims[i][1]) for i in idx))
+        ensure_tuned(ctx, 2 ** lgN, np.complex64, sum(len(sims[i][1]) for i in idx), retune)
     tune_s = time.perf_counter() - t0
-    fused_run()  # warm: plans, tables
+    run_mixed(ctx, mixed)  # warm: plans, tables
     t0 = time.perf_counter()
-    fused = fused_run()
+    rows = run_mixed(ctx, mixed)
     tf = time.perf_counter() - t0
+    fused = dict(enumerate(rows))
+    run_mixed(ctx, mixed, use_graph=True)
+    t0 = time.perf_counter()
+    grows = run_mixed(ctx, mixed, use_graph=True)
+    tg = time.perf_counter() - t0
+    graph_equal = all(np.array_equal(a, b) for a, b in zip(grows, rows))
     t0 = time.perf_counter()
     err_num = err_den = 0.0
     for i, (lgN, srcs) in enumerate(sims):
@@ -182,7 +199,8 @@ def uneven_mix(ravesim, ctx, n_sims=256):
             "fused_gsample_steps_per_s": round(samples * steps / tf / 1e9, 2),
             "plain_c128_one_at_a_time_sims_per_hour": round(n_sims / tp * 3600, 1),
             "speedup": round(tp / tf, 2), "rel_l2_fused_vs_plain_c128": math.sqrt(err_num / err_den),
-            "fused_s": round(tf, 4), "plain_s": round(tp, 4), "tune_s": round(tune_s, 2),
+            "fused_s": round(tf, 4), "fused_graph_s": round(tg, 4), "graph_rows_equal_eager": graph_equal,
+            "plain_s": round(tp, 4), "tune_s": round(tune_s, 2),
             "sims_per_size": {f"2^{k}": len(v) for k, v in sorted(by_n.items())}}
 
 
@@ -306,9 +324,12 @@ def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_e
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     st = srv.stats()
-    # error vs FP64 per request: the smallest requests served again in complex128
+    # error vs FP64 per request: every request up to 2^26 and the 2^28
+    # request with the fewest energies served again in complex128
     errs = []
-    for k in sorted(range(len(arrivals)), key=lambda k: arrivals[k][1].N)[:3]:
+    big = sorted((k for k in range(len(arrivals)) if arrivals[k][1].N == 2 ** 28),
+                 key=lambda k: len(arrivals[k][1].energies_ev))[:1]
+    for k in [k for k in range(len(arrivals)) if arrivals[k][1].N <= 2 ** 26] + big:
         r = arrivals[k][1]
         ref = srv.result(srv.submit(phase_sweep(r.N, r.energies_ev, n_steps=8, slices=64,
                                                 dtype=np.complex128)), timeout_s=600).rows
@@ -318,10 +339,15 @@ def serving_run(ravesim, ctx, qps, duration_s, seed=7, sizes=(22, 24, 26), max_e
     srv.close()
     lat = np.array([r.latency_s for r in res])
     sims = sum(ne for _, ne in ids)
-    return {"qps": qps, "requests": len(res), "sims": sims, "duration_s": round(wall, 2),
+    by_size = {}
+    for (at, req), r in zip(arrivals, res):
+        by_size.setdefault(f"2^{int(math.log2(req.N))}", []).append(round(r.latency_s, 3))
+    return {"qps": qps, "seed": seed, "requests": len(res), "sims": sims, "duration_s": round(wall, 2),
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "sims_per_hour": round(sims / wall * 3600, 1),
+            "requests_by_size": [{"N": f"2^{int(math.log2(q.N))}", "energies": len(q.energies_ev), "at_s": round(at, 2),
+                                  "latency_s": round(r.latency_s, 3)} for (at, q), r in zip(arrivals, res)],
             "mix": f"N in 2^{list(sizes)}, 1-{max_energies} energies, 8 phase steps, 650 steps per energy",
             "jobs_per_launch": round(st["jobs_per_launch"], 2), "max_resident_jobs": st["max_resident_jobs"],
             "error_vs_fp64": errs}
@@ -376,6 +402,14 @@ def cpu_baseline(N, budget_s=25.0, workers=1):
                       f"complex128, scipy.fft {workers} worker(s), {per_step:.2f} s/step, extrapolated x194 steps per sim"}
 
 
+T_START = time.perf_counter()
+
+
+def progress(what):
+    """One line per finished leg on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {time.perf_counter() - T_START:7.1f} s  {what}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -388,18 +422,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the digit-group autotuner")
     ap.add_argument("--split", default="", help="fixed digit-group split 'lgR,lgA' instead of tuning")
+    ap.add_argument("--tune-table", default=str(TUNE_TABLE),
+                    help="tuning table to load (splits and column tiles per N, precision and batch size); "
+                         "configurations it lacks are tuned in the run")
+    ap.add_argument("--retune", action="store_true", help="tune every configuration again, ignoring the table")
+    ap.add_argument("--tune-save", default="", help="write the context's tuning table here at the end")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_talbot_lau_n26.json"))
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparisons")
     ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
     ap.add_argument("--no-simdir", action="store_true", help="skip the sim-dir (PCIe-inclusive) run")
-    ap.add_argument("--serve-qps", type=float, default=1.0,
-                    help="serving run arrival rate in requests/s (0 = skip); the mix saturates near 1.7/s")
+    ap.add_argument("--serve-qps", type=float, default=0.2,
+                    help="serving run arrival rate in requests/s (0 = skip)")
     ap.add_argument("--serve-seconds", type=float, default=30.0)
-    ap.add_argument("--serve-sizes", default="22,24,26",
-                    help="log2 grid sizes of served requests (the full mix: 22,26,28,30)")
-    ap.add_argument("--serve-max-energies", type=int, default=4, help="energies per request: 1..this (full: 32)")
-    ap.add_argument("--serve-seed", type=int, default=7, help="seed of the arrival process")
+    ap.add_argument("--serve-sizes", default="22,26,28,30", help="log2 grid sizes of served requests")
+    ap.add_argument("--serve-max-energies", type=int, default=8, help="energies per request: 1..this")
+    ap.add_argument("--serve-seed", type=int, default=32,
+                    help="seed of the arrival process (32: eight requests from 2^22 to 2^30, up to 8 energies, "
+                         "about 90 s of device work)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the timed run, copy one field this many times (calibrates HBM counters)")
     ap.add_argument("--vendor-fft-only", type=int, default=0, help=argparse.SUPPRESS)
@@ -423,12 +463,17 @@ def main():
     ctx = ravesim.Context(0)
     kw = dict(det_size=1e-3, det_px=1e-6, det_py=1.0, z_detector=zdet, dtype=dtype, plain=args.plain)
 
+    table = {"path": None, "entries": 0}
+    if args.tune_table and Path(args.tune_table).exists() and not args.retune:
+        table = {"path": str(Path(args.tune_table).relative_to(ROOT)) if Path(args.tune_table).is_relative_to(ROOT)
+                 else args.tune_table, "entries": ctx.tune_load(args.tune_table)}
     if args.split:
         lgR, lgA = (int(v) for v in args.split.split(","))
         ctx.set_split(N, dtype, lgR, lgA)
         tuned = {"lgR": lgR, "lgA": lgA, "fixed": True}
     else:
-        tuned = {"default": True} if args.plain or args.no_tune else ctx.autotune(N, dtype)
+        tuned = {"default": True} if args.plain or args.no_tune else ensure_tuned(ctx, N, dtype, 1, args.retune)
+    progress(f"split for the chain: {tuned}")
     for _ in range(args.warmup):
         run_batch(ctx, N, 2e-9, members, **kw)
     ctx.synchronize()
@@ -448,28 +493,44 @@ def main():
     sims_per_hour = args.steps / el * 3600.0
     gss = N * steps_per_sim * args.steps / el / 1e9
 
-    dom = max(stats, key=lambda k: stats[k][1])
+    # Roofline kernel: the top column pass, the one pass of a step that
+    # covers the whole 512 MiB field (twice the 256 MiB last-level cache),
+    # so its reads and writes stream from and to HBM; the middle and row
+    # passes run over cache-sized spans and read largely from that cache.
+    # Beside it: a streaming copy of the same field size in the same run.
+    dom = "top" if stats.get("top", (0,))[0] else max(stats, key=lambda k: stats[k][1])
     launches, tot_ms, byts = stats[dom]
     per_launch_ms = tot_ms / max(launches, 1)
     per_launch_bytes = byts / max(launches, 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    traffic = None
+    copy_gbs = ctx.copy_bandwidth(N * (8 if args.dtype == "c8" else 16), reps=20)
+    split_now = None if args.plain or args.no_tune else [tuned.get("lgR"), tuned.get("lgA"), tuned.get("tileA", 0),
+                                                          tuned.get("tileB", 0)]
+    traffic, traffic_split = None, None
     tp = Path(args.traffic)
     if tp.exists():
         try:
-            rec = json.loads(tp.read_text()).get(dom)
-            traffic = rec.get("hbm_bytes_per_launch") if rec else None
+            prof = json.loads(tp.read_text())
+            traffic_split = prof.get("_split")
+            rec = prof.get(dom)
+            # counters taken at another split describe other kernels
+            if rec and traffic_split == split_now:
+                traffic = rec.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "achievable": round(copy_gbs, 1), "frac_of_achievable": round(achieved / copy_gbs, 4),
                 "kernel": f"{dom} pass", "launches": launches, "timing_stride": TIMING_STRIDE,
                 "avg_ms": round(per_launch_ms, 4),
-                "bytes_per_launch": per_launch_bytes,
+                "bytes_per_launch": per_launch_bytes, "traffic_profiled_at_split": traffic_split,
                 "passes": {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
+    progress(f"timed chain: {sims_per_hour:.0f} sims/h; top pass {achieved:.0f} GB/s, copy {copy_gbs:.0f} GB/s")
     batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx)
+    if batch:
+        progress(f"polychromatic batch: {batch['speedup']}x")
     simdir = None
     if not (args.no_simdir or args.plain):
         try:
@@ -477,9 +538,11 @@ def main():
             simdir["vs_device_resident"] = round(simdir["sims_per_hour"] / sims_per_hour, 4)
         except Exception as e:  # a comparison only: report why it is missing
             simdir = {"error": repr(e)}
+        progress("sim-dir boundary run")
     mix = None
     if not (args.no_batch or args.plain) and args.mix_sims > 0:
-        mix = uneven_mix(ravesim, ctx, args.mix_sims)
+        mix = uneven_mix(ravesim, ctx, args.mix_sims, args.retune)
+        progress(f"uneven mix: fused {mix['fused_s']} s, graph {mix['fused_graph_s']} s, {mix['speedup']}x")
     ours_step_ms = ms_per_step / steps_per_sim
     vendor = None
     if not args.no_rocfft and not args.plain:
@@ -489,6 +552,7 @@ def main():
             vendor = {"error": repr(e)}
         if vendor and "ms_per_propagation_step" in vendor:
             vendor["this_library_ms_per_step_incl_grating"] = round(ours_step_ms, 4)
+        progress("vendor FFT comparison")
     serving = None
     if args.serve_qps > 0 and not args.plain:
         serving = serving_run(ravesim, ctx, args.serve_qps, args.serve_seconds,
@@ -497,6 +561,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        progress("CPU baseline starts (about 40 s)")
         cpu = cpu_baseline(N)
         # the same with every visible core on the FFTs (at most 16, the box's
         # share), a reported extra: big-wave itself runs scipy.fft single-threaded
@@ -524,8 +589,11 @@ def main():
         "vendor_fft": vendor,
         "serving": serving,
         "simdir_end_to_end": simdir,
+        "tuning": {"table": table, "tuned_in_run": TUNE_LOG, "tune_s": round(sum(t["s"] for t in TUNE_LOG), 3)},
         "detector_checksum": float(np.sum(out)),
     }
+    if args.tune_save:
+        ctx.tune_save(args.tune_save)
     print(json.dumps(line), flush=True)
     if args.copy_probe:
         a = ravesim.Field(ctx, N, 2e-9, dtype=dtype)

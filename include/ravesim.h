@@ -271,7 +271,8 @@ typedef struct rs_batch_desc {
   int32_t nr_outputs;      /* accumulator rows */
   int32_t max_resident;    /* members resident at once, one launch per pass covering them; 0 = as many
                               as fit the 256 MiB last-level cache together (at least one), bounded by memory */
-  int32_t use_graph;       /* capture each group's launch sequence in a hipGraph */
+  int32_t use_graph;       /* capture the whole run, every lane, as one hipGraph (lanes fork from and
+                              join the context stream through events), instantiate and launch it */
   int32_t lanes;           /* streams that groups alternate over (1..4); 0 = 2 with the default
                               grouping, 1 otherwise */
   int32_t spans;           /* three-group plans: run each propagation's middle, row and middle passes
@@ -283,6 +284,35 @@ typedef struct rs_batch_desc {
 /* Run every member; out[nr_outputs * n_px] receives sum over members of
  * weight * detector row (FP64).  Only single-phase-step setups. */
 int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *desc, double *out, int64_t n_out);
+
+/* One simulation of a mixed batch: its own grid, detector and gratings; its
+ * members (energies, source points) add weight * detector row into its own
+ * output row (their out_index is ignored). */
+typedef struct rs_batch_sim {
+  rs_grid grid;
+  rs_detector det;
+  double z_detector;
+  int32_t nr_gratings;
+  int32_t nr_members;
+  const rs_batch_member *members;
+  int64_t out_offset;      /* first element of its n_px values in `out` */
+} rs_batch_sim;
+
+typedef struct rs_mixed_desc {
+  int32_t dtype;
+  int32_t mode;
+  int32_t nr_sims;
+  int32_t use_graph;       /* as rs_batch_desc.use_graph, over the whole mixed run */
+  const rs_batch_sim *sims;
+} rs_mixed_desc;
+
+/* Simulations of different grid sizes in one call (the reference runs one
+ * fastwave process per source point, fast-wave/Readme.md:21-34): those that
+ * share N, dx, chunk, detector and element count form one lockstep part, and
+ * the parts' launches interleave over the context's two lanes between one
+ * argument upload and one synchronisation; every simulation's row is what
+ * rs_batch_run gives it.  Single-phase-step setups, one precision. */
+int rs_batch_run_mixed(rs_ctx *ctx, const rs_mixed_desc *desc, double *out, int64_t n_out);
 
 /* ------------------------------------------------------------------ */
 /* serving (no counterpart in the reference, which runs one process per */
@@ -390,6 +420,12 @@ int rs_tune_load(rs_ctx *ctx, const char *path, int32_t *n_loaded);
 int rs_set_timing(rs_ctx *ctx, int enable);
 /* Copy min(n, 4) class records. */
 int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n);
+
+/* Streaming-copy bandwidth of this device (the achievable figure beside a
+ * roofline): `reps` copies of `bytes` between two fresh device buffers with
+ * the library's copy kernel, timed with HIP events on the context stream;
+ * *gbs = 2 * bytes * reps / time (read + write) in GB/s. */
+int rs_copy_bandwidth(rs_ctx *ctx, int64_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

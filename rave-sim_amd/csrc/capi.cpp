@@ -711,6 +711,136 @@ int rs_get_pass_stats(rs_ctx *ctx, rs_pass_stat *out, int n) {
   return RS_OK;
 }
 
+namespace {
+// member program of one (energy, source) batch member
+void build_batch_member(const rs_grid &grid, const rs_detector &det, double z_detector, int dtype, int nr_gratings,
+                        const rs_batch_member &bm, int out_index, rs::Member &m) {
+  rs::SimConfig c;
+  c.N = grid.N;
+  c.dx = grid.dx;
+  c.chunk_size = grid.chunk_size > 0 ? grid.chunk_size : grid.N;
+  c.f64 = dtype == RS_C16;
+  c.z_detector = z_detector;
+  c.detector_size = det.size;
+  c.px = det.pixel_x;
+  c.py = det.pixel_y;
+  c.wl = bm.wl;
+  c.source.kind = bm.source_kind == 1 ? rs::SourceCfg::Plane : rs::SourceCfg::Point;
+  c.source.x = bm.source_x;
+  c.source.z = bm.source_z;
+  if (nr_gratings > 0 && !bm.gratings) throw rs::Error(RS_ERR_INVALID, "member without gratings");
+  if (!bm.cutoff_angles) throw rs::Error(RS_ERR_INVALID, "member without cutoff angles");
+  for (int k = 0; k < nr_gratings; ++k) c.elements.push_back(grating_from(&bm.gratings[k]));
+  for (int k = 0; k <= nr_gratings; ++k) c.cutoff_angles.push_back(bm.cutoff_angles[k]);
+  const int nsteps = rs::build_member(c, m, out_index, false);
+  if (nsteps != 1) throw rs::Error(RS_ERR_UNSUPPORTED, "batched members must not be phase stepped");
+  m.weight = bm.weight;
+}
+}  // namespace
+
+int rs_copy_bandwidth(rs_ctx *ctx, int64_t bytes, int reps, double *gbs) {
+  if (!ctx || !gbs) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
+  if (bytes < 8 || bytes % 8 || reps < 1) return fail(RS_ERR_INVALID, "bytes must be a positive multiple of 8, reps >= 1");
+  return guarded([&] {
+    rs::Context &c = *ctx->ctx;
+    rs::hip_check(hipSetDevice(c.device), "hipSetDevice");
+    rs::DevBuf a((size_t)bytes), b((size_t)bytes);
+    const int64_t n = bytes / 8;
+    rs::hip_check(rs::launch_copy<float>(b.p, a.p, n, c.stream), "copy");  // warm
+    hipEvent_t e0, e1;
+    rs::hip_check(hipEventCreate(&e0), "event");
+    rs::hip_check(hipEventCreate(&e1), "event");
+    rs::hip_check(hipEventRecord(e0, c.stream), "event record");
+    for (int r = 0; r < reps; ++r) rs::hip_check(rs::launch_copy<float>(b.p, a.p, n, c.stream), "copy");
+    rs::hip_check(hipEventRecord(e1, c.stream), "event record");
+    rs::hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    rs::hip_check(hipEventElapsedTime(&ms, e0, e1), "event elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  });
+}
+
+int rs_batch_run_mixed(rs_ctx *ctx, const rs_mixed_desc *d, double *out, int64_t n_out) {
+  if (!ctx || !d || !out) return fail(RS_ERR_INVALID, "NULL argument");
+  if (reserved(ctx)) return fail_reserved();
+  if (d->nr_sims < 1 || !d->sims) return fail(RS_ERR_INVALID, "mixed batch has no simulations");
+  if (d->dtype != RS_C8 && d->dtype != RS_C16) return fail(RS_ERR_INVALID, "dtype must be RS_C8 or RS_C16");
+  if (d->mode != RS_MODE_FUSED && d->mode != RS_MODE_PLAIN) return fail(RS_ERR_INVALID, "unknown mode");
+  return guarded([&] {
+    // parts: simulations that share grid, detector and the operation
+    // sequence of their member programs (slice counts, cutoffs, substrates)
+    struct Part {
+      const rs_batch_sim *first;
+      std::string shape;
+      std::vector<int> sims;
+      std::vector<rs::Member> members;
+    };
+    auto shape_of = [](const rs::Member &m) {
+      std::string sh;
+      for (const auto &o : m.ops) sh.push_back((char)('A' + (int)o.kind));
+      return sh;
+    };
+    std::vector<Part> parts;
+    auto same = [](const rs_batch_sim &a, const rs_batch_sim &b) {
+      return a.grid.N == b.grid.N && a.grid.dx == b.grid.dx && a.grid.chunk_size == b.grid.chunk_size &&
+             a.det.size == b.det.size && a.det.pixel_x == b.det.pixel_x && a.det.pixel_y == b.det.pixel_y &&
+             a.z_detector == b.z_detector && a.nr_gratings == b.nr_gratings;
+    };
+    for (int i = 0; i < d->nr_sims; ++i) {
+      const rs_batch_sim &s = d->sims[i];
+      if (s.nr_members < 1 || !s.members) throw rs::Error(RS_ERR_INVALID, "simulation without members");
+      if (s.grid.N < 64 || (s.grid.N & (s.grid.N - 1)) != 0 || s.grid.N > (int64_t(1) << 30))
+        throw rs::Error(RS_ERR_UNSUPPORTED, "N must be a power of two in [2^6, 2^30]");
+      const int64_t npx = rs::detector_pixels(s.det.size, s.det.pixel_x);
+      if (s.out_offset < 0 || s.out_offset + npx > n_out) throw rs::Error(RS_ERR_INVALID, "output buffer too small");
+      std::vector<rs::Member> ms(s.nr_members);
+      for (int k = 0; k < s.nr_members; ++k)
+        build_batch_member(s.grid, s.det, s.z_detector, d->dtype, s.nr_gratings, s.members[k], 0, ms[k]);
+      const std::string sh = shape_of(ms[0]);
+      for (const auto &m : ms)
+        if (shape_of(m) != sh) throw rs::Error(RS_ERR_INVALID, "the members of a simulation differ in structure");
+      Part *p = nullptr;
+      for (auto &q : parts)
+        if (same(*q.first, s) && q.shape == sh) p = &q;
+      if (!p) {
+        parts.push_back(Part{&s, sh, {}, {}});
+        p = &parts.back();
+      }
+      const int row = (int)p->sims.size();
+      p->sims.push_back(i);
+      for (auto &m : ms) {
+        m.out_base = row;
+        p->members.push_back(std::move(m));
+      }
+    }
+    std::vector<rs::MixedPart> mp;
+    for (auto &p : parts) {
+      const rs_batch_sim &s = *p.first;
+      rs::MixedPart x;
+      x.rc.N = s.grid.N;
+      x.rc.dx = s.grid.dx;
+      x.rc.f64 = d->dtype == RS_C16;
+      x.rc.mode = d->mode;
+      x.rc.det_size = s.det.size;
+      x.rc.det_px = s.det.pixel_x;
+      x.rc.det_py = s.det.pixel_y;
+      x.rc.z_det = s.z_detector;
+      x.rc.n_out = (int)p.sims.size();
+      x.members = &p.members;
+      mp.push_back(x);
+    }
+    const std::vector<std::vector<double>> rows = rs::run_mixed(*ctx->ctx, mp, d->use_graph != 0);
+    for (size_t q = 0; q < parts.size(); ++q) {
+      const int64_t npx = rs::detector_pixels(parts[q].first->det.size, parts[q].first->det.pixel_x);
+      for (size_t r = 0; r < parts[q].sims.size(); ++r)
+        std::memcpy(out + d->sims[parts[q].sims[r]].out_offset, rows[q].data() + r * npx, (size_t)npx * sizeof(double));
+    }
+  });
+}
+
 int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out) {
   if (!ctx || !d || !out) return fail(RS_ERR_INVALID, "NULL argument");
   if (reserved(ctx)) return fail_reserved();
@@ -723,24 +853,7 @@ int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out
     for (int i = 0; i < d->nr_members; ++i) {
       const rs_batch_member &bm = d->members[i];
       if (bm.out_index < 0 || bm.out_index >= d->nr_outputs) throw rs::Error(RS_ERR_INVALID, "out_index out of range");
-      rs::SimConfig c;
-      c.N = d->grid.N;
-      c.dx = d->grid.dx;
-      c.chunk_size = d->grid.chunk_size > 0 ? d->grid.chunk_size : d->grid.N;
-      c.f64 = d->dtype == RS_C16;
-      c.z_detector = d->z_detector;
-      c.detector_size = d->det.size;
-      c.px = d->det.pixel_x;
-      c.py = d->det.pixel_y;
-      c.wl = bm.wl;
-      c.source.kind = bm.source_kind == 1 ? rs::SourceCfg::Plane : rs::SourceCfg::Point;
-      c.source.x = bm.source_x;
-      c.source.z = bm.source_z;
-      for (int k = 0; k < d->nr_gratings; ++k) c.elements.push_back(grating_from(&bm.gratings[k]));
-      for (int k = 0; k <= d->nr_gratings; ++k) c.cutoff_angles.push_back(bm.cutoff_angles[k]);
-      const int nsteps = rs::build_member(c, ms[i], bm.out_index, false);
-      if (nsteps != 1) throw rs::Error(RS_ERR_UNSUPPORTED, "batched members must not be phase stepped");
-      ms[i].weight = bm.weight;
+      build_batch_member(d->grid, d->det, d->z_detector, d->dtype, d->nr_gratings, bm, bm.out_index, ms[i]);
     }
     rs::RunConfig rc;
     rc.N = d->grid.N;
@@ -753,7 +866,7 @@ int rs_batch_run(rs_ctx *ctx, const rs_batch_desc *d, double *out, int64_t n_out
     rc.z_det = d->z_detector;
     rc.n_out = d->nr_outputs;
     rc.max_resident = d->max_resident;
-    rc.use_graph = d->use_graph != 0;
+    rc.use_graph = rc.whole_graph = d->use_graph != 0;
     if (d->lanes < 0 || d->lanes > 4) throw rs::Error(RS_ERR_INVALID, "lanes must be 0..4");
     rc.lanes = d->lanes;
     if (d->spans < 0) throw rs::Error(RS_ERR_INVALID, "spans must be >= 0");

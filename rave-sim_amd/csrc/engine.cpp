@@ -614,162 +614,226 @@ int default_group(size_t bytes_per_member, int64_t n_members) {
   return (int)std::max<size_t>(1, std::min<size_t>(g, 65535));
 }
 
-template <typename S>
-std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
-  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
-  const int64_t npx = detector_pixels(rc.det_size, rc.det_px);
-  const std::vector<int64_t> hb = detector_bounds(rc.N, rc.dx, rc.det_size, rc.det_px);
-  DevBuf dbounds(hb.size() * sizeof(int64_t));
-  hip_check(hipMemcpyAsync(dbounds.p, hb.data(), hb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream),
-            "bounds upload");
-  const size_t nacc = (size_t)rc.n_out * std::max<int64_t>(npx, 0);
-
+// One lockstep run of members that share N, precision, detector and
+// element structure: its plan, groups, buffers and accumulators.  A call of
+// run_members is one BatchRun; run_mixed holds one per distinct
+// configuration and enqueues them all between one argument upload and one
+// synchronisation.
+template <typename S> struct BatchRun {
+  Context &ctx;
+  RunConfig rc;
+  std::vector<Member> &members;
+  int64_t npx = 0;
+  size_t nacc = 0, fbytes = 0, per_member = 1, width = 0;
   bool needs_snap = false, needs_hist = false;
-  for (auto &m : members)
-    for (auto &o : m.ops) {
-      needs_snap |= o.kind == OpKind::Restore;
-      needs_hist |= o.kind == OpKind::HSave;
-    }
-  const size_t per_member = 1 + (needs_snap ? 1 : 0) + (needs_hist ? 1 : 0);  // buffers
-
-  const size_t fbytes = (size_t)rc.N * 2 * sizeof(S);
-  int group = rc.max_resident > 0 ? rc.max_resident : (int)members.size();
-  if (rc.max_resident <= 0 && !rc.fields)
-    group = default_group(fbytes * per_member, (int64_t)members.size());
-  if (rc.fields) group = (int)members.size();
-  group = std::max(1, std::min(group, 65535));
-
-  // Lanes: with the default grouping and more than one group, groups
-  // alternate between the context's two streams, so that a VALU-bound pass of
-  // one group runs beside a memory-bound pass of the other; the resident
-  // fields are split between the lanes.  One lane when timing per pass, under
-  // graph capture, with caller fields or an explicit group size.
-  int lanes = 1;
-  if (!rc.fields && !ctx.timing && !rc.use_graph && members.size() >= 2) {
-    if (rc.lanes > 0) lanes = rc.lanes;
-    else if (rc.max_resident <= 0) lanes = 2;
-  }
-  lanes = std::max(1, std::min({lanes, 4, (int)members.size()}));
-  if (lanes > 1 && rc.max_resident <= 0) group = std::max(1, group / lanes);
-  Plan &plan = ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535), rc.tileA,
-                        rc.tileB);
-  std::vector<hipStream_t> lane_st(lanes);
-  for (int l = 0; l < lanes; ++l) lane_st[l] = ctx.lane_stream(l);
-
-  // per-lane accumulators (summed in lane order on the host)
+  int group = 1, lanes = 1, spans = 1;
+  Plan *plan = nullptr;
+  DevBuf dbounds;
   std::vector<DevBuf> dacc;
-  for (int l = 0; l < lanes; ++l) {
-    dacc.emplace_back(std::max<size_t>(nacc, 1) * sizeof(double));
-    hip_check(launch_fill_f64((double *)dacc.back().p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
-  }
-
-  // Field (and snapshot) buffers of one group per lane, reused by every group
-  // of the lane: each member program starts by generating or uploading its
-  // field.  All groups are compiled first, their argument records go to the
-  // device in one copy, and the launches are enqueued back to back.
-  std::vector<std::vector<DevBuf>> fbuf(lanes), sbuf(lanes), hbuf(lanes);
-  const size_t width = std::min(members.size(), (size_t)group);
-  for (int l = 0; l < lanes; ++l) {
-    if (!rc.fields)
-      for (size_t i = 0; i < width; ++i) fbuf[l].emplace_back(fbytes);
-    if (needs_snap)
-      for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
-    if (needs_hist)
-      for (size_t i = 0; i < width; ++i) hbuf[l].emplace_back(fbytes);
-  }
-  // sample elements: device copies of the members' pixel tables for this run,
-  // pointed at by their SAMPLE masks
+  std::vector<std::vector<DevBuf>> fbuf, sbuf, hbuf;
   std::vector<DevBuf> sample_bufs;
-  auto upload_d = [&](const std::vector<double> &h) -> const double * {
-    sample_bufs.emplace_back(std::max<size_t>(h.size(), 1) * sizeof(double));
-    hip_check(hipMemcpyAsync(sample_bufs.back().p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                             ctx.stream),
-              "sample upload");
-    return (const double *)sample_bufs.back().p;
-  };
-  for (auto &m : members) {
-    if (m.samples.empty()) continue;
-    std::vector<const double *> rx, db, f;
-    for (auto &sr : m.samples) {
-      rx.push_back(upload_d(sr.rx));
-      db.push_back(upload_d(sr.db));
-      f.push_back(upload_d(sr.f));
-    }
-    for (auto &op : m.ops) {
-      if (op.sample < 0) continue;
-      const SampleRows &sr = m.samples.at((size_t)op.sample);
-      op.x.s_rx = rx[(size_t)op.sample];
-      op.x.s_db = db[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
-      op.x.s_f = f[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
-      op.x.s_len = sr.len;
-    }
-  }
-  // the member's host arrays need not outlive the copies
-  if (!sample_bufs.empty()) hip_check(hipStreamSynchronize(ctx.stream), "sample upload sync");
-
-  // spans of the middle-row-middle sequence (three-group plans): a group
-  // whose fields overflow the last-level cache runs it over halves, quarters,
-  // ... of the fields that fit
-  int spans = 1;
-  if (rc.mode == 0 && plan.ngroups == 3 && plan.geom_mid().xcd_segs == 1) {
-    if (rc.spans > 0) {
-      if (rc.spans & (rc.spans - 1) || rc.spans > (1 << plan.lgA))
-        throw Error(1, "spans must be a power of two no larger than the top group");
-      spans = rc.spans;
-    } else {
-      const size_t gbytes = (size_t)lanes * width * fbytes;
-      while (spans < (1 << plan.lgA) && gbytes / (size_t)spans > kMallBytes) spans *= 2;
-    }
-  }
-
   std::vector<std::unique_ptr<Exec<S>>> groups;
-  size_t nfa = 0, nda = 0;
-  for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {
-    const size_t g1 = std::min(members.size(), g0 + group);
-    const int lane = (int)(gi % lanes);
-    auto ex = std::make_unique<Exec<S>>(ctx, rc, plan);
-    ex->st = lane_st[lane];
-    ex->spans = spans;
-    for (size_t i = g0; i < g1; ++i) {
-      ex->mem.push_back(&members[i]);
-      ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);
-      if (needs_snap) ex->snap.push_back(sbuf[lane][i - g0].p);
-      if (needs_hist) ex->hist.push_back(hbuf[lane][i - g0].p);
-    }
-    ex->acc = (double *)dacc[lane].p;
-    ex->bounds = (const int64_t *)dbounds.p;
-    ex->npx = npx;
-    ex->compile();
-    nfa += ex->fargs.size();
-    nda += ex->dargs.size();
-    groups.push_back(std::move(ex));
-  }
-  // Grating decisions shared by several passes: computed once, read as bits
-  // (N/8 bytes per mask).  Only when the resident fields overflow the
-  // last-level cache anyway: when they fit, the extra stream evicts them and
-  // costs more than the decisions it saves.
   MaskCache<S> masks;
-  const size_t resident = (size_t)lanes * width * fbytes * per_member;
-  if (rc.mode == 0 && resident > kMallBytes)
-    for (auto &ex : groups) masks.collect(*ex);
-  masks.fill(rc.N, ctx.stream);
+  std::vector<std::vector<double>> accs;
+
+  BatchRun(Context &c, const RunConfig &r, std::vector<Member> &m) : ctx(c), rc(r), members(m) {}
+
+  // plan, buffers, compiled groups and the shared grating decisions, with
+  // the uploads on the context stream; `lanes_max` bounds the lanes (0: no
+  // bound beyond the default)
+  void setup(int lanes_max = 0) {
+    npx = detector_pixels(rc.det_size, rc.det_px);
+    const std::vector<int64_t> hb = detector_bounds(rc.N, rc.dx, rc.det_size, rc.det_px);
+    dbounds = DevBuf(hb.size() * sizeof(int64_t));
+    hip_check(hipMemcpyAsync(dbounds.p, hb.data(), hb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream),
+              "bounds upload");
+    nacc = (size_t)rc.n_out * std::max<int64_t>(npx, 0);
+    for (auto &m : members)
+      for (auto &o : m.ops) {
+        needs_snap |= o.kind == OpKind::Restore;
+        needs_hist |= o.kind == OpKind::HSave;
+      }
+    per_member = 1 + (needs_snap ? 1 : 0) + (needs_hist ? 1 : 0);  // buffers
+    fbytes = (size_t)rc.N * 2 * sizeof(S);
+    group = rc.max_resident > 0 ? rc.max_resident : (int)members.size();
+    if (rc.max_resident <= 0 && !rc.fields) group = default_group(fbytes * per_member, (int64_t)members.size());
+    if (rc.fields) group = (int)members.size();
+    group = std::max(1, std::min(group, 65535));
+
+    // Lanes: with the default grouping and more than one group, groups
+    // alternate between the context's two streams, so that a VALU-bound pass
+    // of one group runs beside a memory-bound pass of the other; the
+    // resident fields are split between the lanes.  One lane when timing per
+    // pass, with caller fields or an explicit group size.
+    lanes = 1;
+    if (!rc.fields && !ctx.timing && !(rc.use_graph && !rc.whole_graph) && members.size() >= 2) {
+      if (rc.lanes > 0) lanes = rc.lanes;
+      else if (rc.max_resident <= 0) lanes = 2;
+    }
+    lanes = std::max(1, std::min({lanes, 4, (int)members.size()}));
+    if (lanes_max > 0) lanes = std::min(lanes, lanes_max);
+    if (lanes > 1 && rc.max_resident <= 0) group = std::max(1, group / lanes);
+    plan = &ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535), rc.tileA,
+                     rc.tileB);
+
+    // per-lane accumulators (summed in lane order on the host)
+    for (int l = 0; l < lanes; ++l) {
+      dacc.emplace_back(std::max<size_t>(nacc, 1) * sizeof(double));
+      hip_check(launch_fill_f64((double *)dacc.back().p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
+    }
+
+    // Field (and snapshot) buffers of one group per lane, reused by every
+    // group of the lane: each member program starts by generating or
+    // uploading its field.
+    fbuf.resize(lanes);
+    sbuf.resize(lanes);
+    hbuf.resize(lanes);
+    width = std::min(members.size(), (size_t)group);
+    for (int l = 0; l < lanes; ++l) {
+      if (!rc.fields)
+        for (size_t i = 0; i < width; ++i) fbuf[l].emplace_back(fbytes);
+      if (needs_snap)
+        for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
+      if (needs_hist)
+        for (size_t i = 0; i < width; ++i) hbuf[l].emplace_back(fbytes);
+    }
+    // sample elements: device copies of the members' pixel tables for this
+    // run, pointed at by their SAMPLE masks
+    auto upload_d = [&](const std::vector<double> &h) -> const double * {
+      sample_bufs.emplace_back(std::max<size_t>(h.size(), 1) * sizeof(double));
+      hip_check(hipMemcpyAsync(sample_bufs.back().p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                               ctx.stream),
+                "sample upload");
+      return (const double *)sample_bufs.back().p;
+    };
+    for (auto &m : members) {
+      if (m.samples.empty()) continue;
+      std::vector<const double *> rx, db, f;
+      for (auto &sr : m.samples) {
+        rx.push_back(upload_d(sr.rx));
+        db.push_back(upload_d(sr.db));
+        f.push_back(upload_d(sr.f));
+      }
+      for (auto &op : m.ops) {
+        if (op.sample < 0) continue;
+        const SampleRows &sr = m.samples.at((size_t)op.sample);
+        op.x.s_rx = rx[(size_t)op.sample];
+        op.x.s_db = db[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
+        op.x.s_f = f[(size_t)op.sample] + (size_t)op.sample_row * 2 * sr.len;
+        op.x.s_len = sr.len;
+      }
+    }
+    // the member's host arrays need not outlive the copies
+    if (!sample_bufs.empty()) hip_check(hipStreamSynchronize(ctx.stream), "sample upload sync");
+
+    // spans of the middle-row-middle sequence (three-group plans): a group
+    // whose fields overflow the last-level cache runs it over halves,
+    // quarters, ... of the fields that fit
+    spans = 1;
+    if (rc.mode == 0 && plan->ngroups == 3 && plan->geom_mid().xcd_segs == 1) {
+      if (rc.spans > 0) {
+        if (rc.spans & (rc.spans - 1) || rc.spans > (1 << plan->lgA))
+          throw Error(1, "spans must be a power of two no larger than the top group");
+        spans = rc.spans;
+      } else {
+        const size_t gbytes = (size_t)lanes * width * fbytes;
+        while (spans < (1 << plan->lgA) && gbytes / (size_t)spans > kMallBytes) spans *= 2;
+      }
+    }
+
+    for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {
+      const size_t g1 = std::min(members.size(), g0 + group);
+      const int lane = (int)(gi % lanes);
+      auto ex = std::make_unique<Exec<S>>(ctx, rc, *plan);
+      ex->st = ctx.lane_stream(lane);
+      ex->spans = spans;
+      for (size_t i = g0; i < g1; ++i) {
+        ex->mem.push_back(&members[i]);
+        ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);
+        if (needs_snap) ex->snap.push_back(sbuf[lane][i - g0].p);
+        if (needs_hist) ex->hist.push_back(hbuf[lane][i - g0].p);
+      }
+      ex->acc = (double *)dacc[lane].p;
+      ex->bounds = (const int64_t *)dbounds.p;
+      ex->npx = npx;
+      ex->compile();
+      groups.push_back(std::move(ex));
+    }
+    // Grating decisions shared by several passes: computed once, read as
+    // bits (N/8 bytes per mask).  Only when the resident fields overflow the
+    // last-level cache anyway: when they fit, the extra stream evicts them
+    // and costs more than the decisions it saves.
+    const size_t resident = (size_t)lanes * width * fbytes * per_member;
+    if (rc.mode == 0 && resident > kMallBytes)
+      for (auto &ex : groups) masks.collect(*ex);
+    masks.fill(rc.N, ctx.stream);
+  }
+
+  void append_args(std::vector<FieldArgs> &fa, std::vector<DetArgs> &da) const {
+    for (auto &ex : groups) {
+      fa.insert(fa.end(), ex->fargs.begin(), ex->fargs.end());
+      da.insert(da.end(), ex->dargs.begin(), ex->dargs.end());
+    }
+  }
+
+  // every group's launches on its lane, argument records at fa / da
+  void enqueue(const FieldArgs *fa, const DetArgs *da) {
+    size_t ofa = 0, oda = 0;
+    for (auto &ex : groups) {
+      ex->enqueue(fa + ofa, da + oda);
+      ofa += ex->fargs.size();
+      oda += ex->dargs.size();
+    }
+  }
+
+  // accumulator download on the context stream (after the lanes joined it)
+  void download() {
+    accs.assign(lanes, std::vector<double>(nacc));
+    for (int l = 0; l < lanes && nacc; ++l)
+      hip_check(hipMemcpyAsync(accs[l].data(), dacc[l].p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+                "acc download");
+  }
+
+  // after the stream drained: pass timings and the scaled detector rows
+  std::vector<double> finish() {
+    for (auto &ex : groups) ex->collect_timing();
+    if (!rc.row_z.empty() && (int)rc.row_z.size() != rc.n_out) throw Error(1, "row_z must have n_out entries");
+    std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
+    std::vector<double> out(nacc);
+    for (int r = 0; r < rc.n_out; ++r) {
+      if (!rc.row_z.empty() && (r == 0 || rc.row_z[(size_t)r] != rc.row_z[(size_t)r - 1]))
+        sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.row_z[(size_t)r]);
+      for (int64_t k = 0; k < npx; ++k) {
+        double v = accs[0][(size_t)r * npx + k];
+        for (int l = 1; l < lanes; ++l) v += accs[l][(size_t)r * npx + k];
+        out[(size_t)r * npx + k] = (((v * sc[2 * k]) * rc.dx) * rc.det_py) / sc[2 * k + 1];
+      }
+    }
+    return out;
+  }
+};
+
+// Enqueue runs (already set up) on the context's lanes between one argument
+// upload and one synchronisation.  whole_graph: the launches of every lane
+// are captured as one hipGraph (the lanes fork from and join the context
+// stream through events inside the capture), instantiated and launched.
+template <typename S>
+void execute_runs(Context &ctx, std::vector<std::unique_ptr<BatchRun<S>>> &runs, bool whole_graph) {
   std::vector<FieldArgs> hfa;
   std::vector<DetArgs> hda;
-  hfa.reserve(nfa);
-  hda.reserve(nda);
-  for (auto &ex : groups) {
-    hfa.insert(hfa.end(), ex->fargs.begin(), ex->fargs.end());
-    hda.insert(hda.end(), ex->dargs.begin(), ex->dargs.end());
+  int lanes = 1;
+  for (auto &r : runs) {
+    r->append_args(hfa, hda);
+    lanes = std::max(lanes, r->lanes);
   }
-  DevBuf dfa(std::max<size_t>(nfa, 1) * sizeof(FieldArgs));
-  DevBuf dda(std::max<size_t>(nda, 1) * sizeof(DetArgs));
-  if (nfa)
-    hip_check(hipMemcpyAsync(dfa.p, hfa.data(), nfa * sizeof(FieldArgs), hipMemcpyHostToDevice, ctx.stream),
+  DevBuf dfa(std::max<size_t>(hfa.size(), 1) * sizeof(FieldArgs));
+  DevBuf dda(std::max<size_t>(hda.size(), 1) * sizeof(DetArgs));
+  if (!hfa.empty())
+    hip_check(hipMemcpyAsync(dfa.p, hfa.data(), hfa.size() * sizeof(FieldArgs), hipMemcpyHostToDevice, ctx.stream),
               "args upload");
-  if (nda)
-    hip_check(hipMemcpyAsync(dda.p, hda.data(), nda * sizeof(DetArgs), hipMemcpyHostToDevice, ctx.stream),
+  if (!hda.empty())
+    hip_check(hipMemcpyAsync(dda.p, hda.data(), hda.size() * sizeof(DetArgs), hipMemcpyHostToDevice, ctx.stream),
               "det args upload");
-  // lanes start after the uploads on the context stream and join it at the end
   std::vector<hipEvent_t> evs;
   auto new_event = [&]() {
     hipEvent_t e = nullptr;
@@ -777,43 +841,80 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
     evs.push_back(e);
     return e;
   };
-  if (lanes > 1) {
-    hipEvent_t ready = new_event();
-    hip_check(hipEventRecord(ready, ctx.stream), "event record");
-    for (int l = 1; l < lanes; ++l) hip_check(hipStreamWaitEvent(lane_st[l], ready, 0), "stream wait");
-  }
-  size_t ofa = 0, oda = 0;
-  for (auto &ex : groups) {
-    ex->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
-    ofa += ex->fargs.size();
-    oda += ex->dargs.size();
-  }
-  for (int l = 1; l < lanes; ++l) {
-    hipEvent_t done = new_event();
-    hip_check(hipEventRecord(done, lane_st[l]), "event record");
-    hip_check(hipStreamWaitEvent(ctx.stream, done, 0), "stream wait");
-  }
-
-  std::vector<std::vector<double>> accs(lanes, std::vector<double>(nacc));
-  for (int l = 0; l < lanes && nacc; ++l)
-    hip_check(hipMemcpyAsync(accs[l].data(), dacc[l].p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
-              "acc download");
-  hip_check(hipStreamSynchronize(ctx.stream), "sync");
-  for (hipEvent_t e : evs) (void)hipEventDestroy(e);
-  for (auto &ex : groups) ex->collect_timing();
-
-  if (!rc.row_z.empty() && (int)rc.row_z.size() != rc.n_out) throw Error(1, "row_z must have n_out entries");
-  std::vector<double> sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.z_det);
-  std::vector<double> out(nacc);
-  for (int r = 0; r < rc.n_out; ++r) {
-    if (!rc.row_z.empty() && (r == 0 || rc.row_z[(size_t)r] != rc.row_z[(size_t)r - 1]))
-      sc = detector_scale(rc.det_size, rc.det_px, rc.det_py, rc.dx, rc.row_z[(size_t)r]);
-    for (int64_t k = 0; k < npx; ++k) {
-      double v = accs[0][(size_t)r * npx + k];
-      for (int l = 1; l < lanes; ++l) v += accs[l][(size_t)r * npx + k];
-      out[(size_t)r * npx + k] = (((v * sc[2 * k]) * rc.dx) * rc.det_py) / sc[2 * k + 1];
+  struct EventsFree {
+    std::vector<hipEvent_t> &e;
+    ~EventsFree() {
+      for (hipEvent_t x : e) (void)hipEventDestroy(x);
     }
+  } events_free{evs};
+  hipGraph_t graph = nullptr;
+  if (whole_graph) hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    // lanes start after the uploads on the context stream and join it at the end
+    if (lanes > 1) {
+      hipEvent_t ready = new_event();
+      hip_check(hipEventRecord(ready, ctx.stream), "event record");
+      for (int l = 1; l < lanes; ++l) hip_check(hipStreamWaitEvent(ctx.lane_stream(l), ready, 0), "stream wait");
+    }
+    size_t ofa = 0, oda = 0;
+    for (auto &r : runs) {
+      r->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
+      for (auto &ex : r->groups) {
+        ofa += ex->fargs.size();
+        oda += ex->dargs.size();
+      }
+    }
+    for (int l = 1; l < lanes; ++l) {
+      hipEvent_t done = new_event();
+      hip_check(hipEventRecord(done, ctx.lane_stream(l)), "event record");
+      hip_check(hipStreamWaitEvent(ctx.stream, done, 0), "stream wait");
+    }
+  } catch (...) {
+    if (whole_graph) {
+      (void)hipStreamEndCapture(ctx.stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+    throw;
   }
+  if (whole_graph) {
+    hip_check(hipStreamEndCapture(ctx.stream, &graph), "end capture");
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(ie, "graph instantiate");
+    const hipError_t le = hipGraphLaunch(ex, ctx.stream);
+    if (le == hipSuccess) hip_check(hipStreamSynchronize(ctx.stream), "graph sync");
+    (void)hipGraphExecDestroy(ex);
+    hip_check(le, "graph launch");
+  }
+  for (auto &r : runs) r->download();
+  hip_check(hipStreamSynchronize(ctx.stream), "sync");
+}
+
+template <typename S>
+std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  std::vector<std::unique_ptr<BatchRun<S>>> runs;
+  runs.push_back(std::make_unique<BatchRun<S>>(ctx, rc, members));
+  runs[0]->setup();
+  execute_runs<S>(ctx, runs, rc.use_graph && rc.whole_graph);
+  return runs[0]->finish();
+}
+
+template <typename S>
+std::vector<std::vector<double>> run_mixed_t(Context &ctx, const std::vector<MixedPart> &parts, bool whole_graph) {
+  hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  std::vector<std::unique_ptr<BatchRun<S>>> runs;
+  for (const MixedPart &p : parts) {
+    RunConfig rc = p.rc;
+    rc.use_graph = whole_graph;
+    rc.whole_graph = whole_graph;
+    runs.push_back(std::make_unique<BatchRun<S>>(ctx, rc, *p.members));
+    runs.back()->setup();
+  }
+  execute_runs<S>(ctx, runs, whole_graph);
+  std::vector<std::vector<double>> out;
+  for (auto &r : runs) out.push_back(r->finish());
   return out;
 }
 
@@ -822,6 +923,17 @@ std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector
 std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   if (members.empty()) return {};
   return rc.f64 ? run_members_t<double>(ctx, rc, members) : run_members_t<float>(ctx, rc, members);
+}
+
+std::vector<std::vector<double>> run_mixed(Context &ctx, const std::vector<MixedPart> &parts, bool whole_graph) {
+  if (parts.empty()) return {};
+  const bool f64 = parts[0].rc.f64;
+  for (const MixedPart &p : parts) {
+    if (p.rc.f64 != f64) throw Error(1, "the parts of a mixed run share one precision");
+    if (!p.members || p.members->empty()) throw Error(1, "a part of a mixed run has no members");
+    if (p.rc.fields) throw Error(1, "mixed runs allocate their own fields");
+  }
+  return f64 ? run_mixed_t<double>(ctx, parts, whole_graph) : run_mixed_t<float>(ctx, parts, whole_graph);
 }
 
 Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps, int members) {

@@ -155,7 +155,10 @@ struct RunConfig {
   int n_out = 1;             // accumulator rows
   std::vector<double> row_z; // detector distance of each row (empty: z_det for all)
   int max_resident = 0;      // members per group; 0 = as many as fit the last-level cache together
+  // capture the whole run, every lane, as one hipGraph (lanes fork from and
+  // join the context stream through events inside the capture)
   bool use_graph = false;
+  bool whole_graph = false;  // set with use_graph by the runners
   int lanes = 0;             // streams groups alternate over; 0 = 2 with the default grouping
   int spans = 0;             // sub-ranges of the middle-row-middle passes (3 groups); 0 = by cache size
   int lgR = -1;              // row length override (autotuner), -1 = tuned or default
@@ -168,6 +171,18 @@ class Context;
 
 // Run all members; returns scaled detector rows, n_out x n_px (row-major).
 std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<Member> &members);
+
+// One lockstep part of a mixed run: members that share N, the detector and
+// the element structure, with their run configuration.
+struct MixedPart {
+  RunConfig rc;
+  std::vector<Member> *members = nullptr;
+};
+// Run parts of different N (one precision) together: one argument upload,
+// the groups of every part interleaved over the context's lanes, one
+// synchronisation; whole_graph captures all of it as one hipGraph.  Returns
+// each part's scaled rows.
+std::vector<std::vector<double>> run_mixed(Context &ctx, const std::vector<MixedPart> &parts, bool whole_graph);
 
 // ----------------------------------------------------------------------------
 // context: device + stream + plan cache

@@ -459,48 +459,14 @@ template <typename S> struct Exec {
   std::vector<hipEvent_t> ev;
   std::vector<std::pair<int, double>> ev_meta;  // class, bytes
 
-  // Upload this program's argument records, enqueue it and wait for it.
-  void execute() {
-    DevBuf dfa(std::max<size_t>(fargs.size(), 1) * sizeof(FieldArgs));
-    DevBuf dda(std::max<size_t>(dargs.size(), 1) * sizeof(DetArgs));
-    if (!fargs.empty())
-      hip_check(hipMemcpyAsync(dfa.p, fargs.data(), fargs.size() * sizeof(FieldArgs), hipMemcpyHostToDevice, st),
-                "args upload");
-    if (!dargs.empty())
-      hip_check(hipMemcpyAsync(dda.p, dargs.data(), dargs.size() * sizeof(DetArgs), hipMemcpyHostToDevice, st),
-                "det args upload");
-    enqueue((const FieldArgs *)dfa.p, (const DetArgs *)dda.p);
-    hip_check(hipStreamSynchronize(st), "run sync");
-    collect_timing();
-  }
-
-  // Enqueue the launches on the context stream with argument records already
+  // Enqueue the launches on the group's stream with argument records already
   // on the device (fa[i] = fargs[i], da[i] = dargs[i]); returns without
-  // waiting unless a graph or a host download needs it.
+  // waiting unless a host download needs it.  Under a whole-run graph
+  // capture (rc.whole_graph) only device work may be enqueued.
   void enqueue(const FieldArgs *fa, const DetArgs *da) {
     const size_t esz = 2 * sizeof(S);
-
-    // graph capture covers the run of launches between host interactions
-    bool capturing = false;
-    hipGraph_t graph = nullptr;
-    auto begin_capture = [&]() {
-      if (rc.use_graph && !capturing) {
-        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-        capturing = true;
-      }
-    };
-    auto end_capture = [&]() {
-      if (capturing) {
-        hip_check(hipStreamEndCapture(st, &graph), "end capture");
-        capturing = false;
-        hipGraphExec_t ex = nullptr;
-        hip_check(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0), "graph instantiate");
-        hip_check(hipGraphLaunch(ex, st), "graph launch");
-        hip_check(hipStreamSynchronize(st), "graph sync");
-        (void)hipGraphExecDestroy(ex);
-        (void)hipGraphDestroy(graph);
-        graph = nullptr;
-      }
+    auto host_step = [&]() {
+      if (rc.whole_graph) throw Error(6, "file uploads and downloads cannot run inside a captured graph");
     };
 
     // optional per-launch event timing (not combined with graph capture).
@@ -533,7 +499,6 @@ template <typename S> struct Exec {
     for (const Launch &l : launches) {
       switch (l.kind) {
         case LKind::Fiber: {
-          begin_capture();
           const int cls = l.row ? PC_ROW : (l.geom.top ? PC_TOP : PC_MID);
           const double ns = (double)(l.geom.span_len ? l.geom.span_len : rc.N);
           hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -547,38 +512,33 @@ template <typename S> struct Exec {
           break;
         }
         case LKind::Xop:
-          begin_capture();
           t_begin(PC_OTHER, 2.0 * (double)rc.N * (double)esz * l.nf);
           hip_check(launch_xop<S>(fa + l.arg_off, l.nf, rc.N, st), "xop");
           t_end();
           break;
         case LKind::Kop:
-          begin_capture();
           t_begin(PC_OTHER, 2.0 * (double)rc.N * (double)esz * l.nf);
           hip_check(launch_kop<S>(fa + l.arg_off, l.nf, l.geom, st), "kop");
           t_end();
           break;
         case LKind::Detect:
-          begin_capture();
           hip_check(launch_detect<S>(da + l.arg_off, l.nf, bounds, npx, st), "detect");
           break;
         case LKind::Copy:
-          begin_capture();
           hip_check(launch_copy<S>(l.dst, l.src, l.n, st), "copy");
           break;
         case LKind::Upload: {
-          end_capture();
+          host_step();
           const Member &m = *mem[l.member];
           if (m.upload_npy.empty()) throw Error(1, "upload op without a source file");
           ctx.stream_npy(m, sizeof(S) == 8, l.n, l.dst, st);
           break;
         }
         case LKind::Fill:
-          begin_capture();
           hip_check(launch_fill_f64((double *)l.dst, l.n, 0.0, st), "fill");
           break;
         case LKind::SaveU: {
-          end_capture();
+          host_step();
           Member *m = mem[l.member];
           const auto &file = l.keypoint ? m->keypoint_file : m->saveu_file;
           if (file) ctx.stream_to_npy(l.src, l.n, sizeof(S) == 8, file(l.step), st);
@@ -586,7 +546,6 @@ template <typename S> struct Exec {
         }
       }
     }
-    end_capture();
   }
 
   // after the stream has drained: add the timed launches to ctx.stats

@@ -7,4 +7,4 @@ Thin Python layer over librsim.so, mirroring big-wave's entry points:
 and a batched runner (ravesim.core.run_batch) for many energies / sources.
 """
 from ._lib import RS_C8, RS_C16, RS_MODE_FUSED, RS_MODE_PLAIN, RsError  # noqa: F401
-from .core import Context, DBAGrating, Field, Grating, Sample, run_batch, energy_to_wavelength  # noqa: F401
+from .core import BatchSim, Context, DBAGrating, Field, Grating, Sample, energy_to_wavelength, run_batch, run_mixed  # noqa: F401

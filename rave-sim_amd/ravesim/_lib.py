@@ -34,7 +34,7 @@ EXPORTS = (
     "rs_field_upload", "rs_field_download", "rs_field_copy", "rs_point_source", "rs_propagate",
     "rs_apply_grating", "rs_apply_dba_grating", "rs_apply_sample", "rs_scale", "rs_square_and_downsample", "rs_square_and_downsample_noalloc",
     "rs_detector_pixels",
-    "rs_detector_bounds", "rs_batch_run", "rs_set_timing", "rs_get_pass_stats", "rs_autotune", "rs_set_split",
+    "rs_detector_bounds", "rs_batch_run", "rs_batch_run_mixed", "rs_set_timing", "rs_get_pass_stats", "rs_copy_bandwidth", "rs_autotune", "rs_set_split",
     "rs_tune_get", "rs_tune_save", "rs_tune_load",
     "rs_server_create", "rs_server_submit", "rs_server_poll", "rs_server_wait", "rs_server_stats",
     "rs_server_destroy",
@@ -103,6 +103,17 @@ class rs_batch_desc(C.Structure):
                 ("use_graph", C.c_int32), ("lanes", C.c_int32), ("spans", C.c_int32)]
 
 
+class rs_batch_sim(C.Structure):
+    _fields_ = [("grid", rs_grid), ("det", rs_detector), ("z_detector", C.c_double),
+                ("nr_gratings", C.c_int32), ("nr_members", C.c_int32),
+                ("members", C.POINTER(rs_batch_member)), ("out_offset", C.c_int64)]
+
+
+class rs_mixed_desc(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("mode", C.c_int32), ("nr_sims", C.c_int32),
+                ("use_graph", C.c_int32), ("sims", C.POINTER(rs_batch_sim))]
+
+
 class rs_request(C.Structure):
     _fields_ = [("grid", rs_grid), ("det", rs_detector), ("z_detector", C.c_double),
                 ("dtype", C.c_int32), ("nr_gratings", C.c_int32), ("nr_energies", C.c_int32),
@@ -158,6 +169,7 @@ def _load() -> C.CDLL:
         "rs_detector_bounds": (C.c_int, [C.c_int64, C.c_double, C.POINTER(rs_detector),
                                          C.POINTER(C.c_int64), C.c_int64]),
         "rs_batch_run": (C.c_int, [P, C.POINTER(rs_batch_desc), C.POINTER(C.c_double), C.c_int64]),
+        "rs_batch_run_mixed": (C.c_int, [P, C.POINTER(rs_mixed_desc), C.POINTER(C.c_double), C.c_int64]),
         "rs_set_timing": (C.c_int, [P, C.c_int]),
         "rs_autotune": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
         "rs_set_split": (C.c_int, [P, C.c_int64, C.c_int, C.c_int, C.c_int]),
@@ -172,6 +184,7 @@ def _load() -> C.CDLL:
         "rs_server_stats": (C.c_int, [P, C.POINTER(C.c_double)]),
         "rs_server_destroy": (C.c_int, [P]),
         "rs_get_pass_stats": (C.c_int, [P, C.POINTER(rs_pass_stat), C.c_int]),
+        "rs_copy_bandwidth": (C.c_int, [P, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -87,6 +87,12 @@ class Context:
         names = ("row", "top", "mid", "other")
         return {n: (a.launches, a.total_ms, a.bytes) for n, a in zip(names, arr)}
 
+    def copy_bandwidth(self, nbytes: int, reps: int = 10) -> float:
+        """Streaming-copy GB/s (read + write) of `reps` device copies of `nbytes`."""
+        g = C.c_double()
+        check(lib.rs_copy_bandwidth(self.handle, int(nbytes), int(reps), C.byref(g)))
+        return g.value
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             lib.rs_ctx_destroy(self._h)
@@ -323,18 +329,7 @@ class BatchMember:
     source_z: float = 0.0
 
 
-def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *,
-              det_size: float, det_px: float, det_py: float, z_detector: float,
-              n_outputs: int = 1, dtype=np.complex64, chunk_size: Optional[int] = None,
-              plain: bool = False, max_resident: int = 0, use_graph: bool = False,
-              lanes: int = 0, spans: int = 0) -> np.ndarray:
-    """Run every member in lockstep batches; returns (n_outputs, n_px) float64
-    with sum over members of weight * detector row.  `spans` (three-group
-    plans): sub-ranges the middle-row-middle passes run over, 0 = by cache size."""
-    if not members:
-        raise ValueError("no members")
-    ng = len(members[0].gratings)
-    keep = []
+def _members_c(members: Sequence[BatchMember], ng: int, keep: list):
     cm = (_lib.rs_batch_member * len(members))()
     for i, m in enumerate(members):
         if len(m.gratings) != ng or len(m.cutoff_angles) != ng + 1:
@@ -350,6 +345,66 @@ def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *
                                      out_index=m.out_index, source_x=m.source_x, source_z=m.source_z,
                                      gratings=C.cast(garr, C.POINTER(_lib.rs_grating)),
                                      cutoff_angles=C.cast(ang, C.POINTER(C.c_double)))
+    keep.append(cm)
+    return cm
+
+
+@dataclass
+class BatchSim:
+    """One simulation of a mixed batch: its own grid and detector; its
+    members' weighted detector rows add into one output row."""
+
+    N: int
+    dx: float
+    members: Sequence[BatchMember]
+    det_size: float
+    det_px: float
+    z_detector: float
+    det_py: float = 1.0
+    chunk_size: Optional[int] = None
+
+
+def run_mixed(ctx: Context, sims: Sequence[BatchSim], *, dtype=np.complex64, plain: bool = False,
+              use_graph: bool = False) -> list:
+    """Simulations of different grid sizes in one call (rs_batch_run_mixed):
+    returns one float64 detector row per simulation, each what run_batch
+    gives it.  use_graph: the whole run, both lanes, as one hipGraph."""
+    if not sims:
+        raise ValueError("no simulations")
+    keep = []
+    cs = (_lib.rs_batch_sim * len(sims))()
+    offs, off = [], 0
+    for i, s in enumerate(sims):
+        ng = len(s.members[0].gratings)
+        cm = _members_c(s.members, ng, keep)
+        det = _lib.rs_detector(size=s.det_size, pixel_x=s.det_px, pixel_y=s.det_py)
+        n = lib.rs_detector_pixels(C.byref(det))
+        cs[i] = _lib.rs_batch_sim(grid=_lib.rs_grid(N=int(s.N), dx=float(s.dx), chunk_size=int(s.chunk_size or s.N)),
+                                  det=det, z_detector=s.z_detector, nr_gratings=ng, nr_members=len(s.members),
+                                  members=C.cast(cm, C.POINTER(_lib.rs_batch_member)), out_offset=off)
+        offs.append((off, n))
+        off += n
+    desc = _lib.rs_mixed_desc(dtype=_lib.RS_C8 if np.dtype(dtype) == np.complex64 else _lib.RS_C16,
+                              mode=_lib.RS_MODE_PLAIN if plain else _lib.RS_MODE_FUSED, nr_sims=len(sims),
+                              use_graph=1 if use_graph else 0, sims=cs)
+    out = np.zeros(max(off, 1), dtype=np.float64)
+    check(lib.rs_batch_run_mixed(ctx.handle, C.byref(desc), out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+    return [out[o:o + n].copy() for o, n in offs]
+
+
+def run_batch(ctx: Context, N: int, dx: float, members: Sequence[BatchMember], *,
+              det_size: float, det_px: float, det_py: float, z_detector: float,
+              n_outputs: int = 1, dtype=np.complex64, chunk_size: Optional[int] = None,
+              plain: bool = False, max_resident: int = 0, use_graph: bool = False,
+              lanes: int = 0, spans: int = 0) -> np.ndarray:
+    """Run every member in lockstep batches; returns (n_outputs, n_px) float64
+    with sum over members of weight * detector row.  `spans` (three-group
+    plans): sub-ranges the middle-row-middle passes run over, 0 = by cache size."""
+    if not members:
+        raise ValueError("no members")
+    ng = len(members[0].gratings)
+    keep = []
+    cm = _members_c(members, ng, keep)
     desc = _lib.rs_batch_desc(
         grid=_lib.rs_grid(N=N, dx=dx, chunk_size=int(chunk_size or N)),
         det=_lib.rs_detector(size=det_size, pixel_x=det_px, pixel_y=det_py),

@@ -336,10 +336,11 @@ def test_pass_timing_samples_every_kth_launch():
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_graph_capture_matches_eager(dtype):
-    """rs_batch_desc.use_graph captures each group's launch sequence in a
-    hipGraph and launches it: three graph runs, with eager runs and
-    synchronisations between them, each equal to the eager result (the same
-    kernels on the same inputs)."""
+    """rs_batch_desc.use_graph captures the whole run as one hipGraph and
+    launches it: three graph runs, with eager runs and synchronisations
+    between them, each equal to the eager result (the same kernels on the
+    same inputs); one lane (explicit group size) and two lanes (default
+    grouping, forked and joined by events inside the capture)."""
     import ravesim
     from ravesim.core import BatchMember, run_batch
     N, dx = 2 ** 16, 2e-9
@@ -349,13 +350,14 @@ def test_graph_capture_matches_eager(dtype):
     ms = [BatchMember(wl=ravesim.energy_to_wavelength(e), gratings=[g], cutoff_angles=[0.2] * 2,
                       source_x=1e-7 * k, source_z=-0.1, weight=0.5 + 0.1 * k)
           for k, e in enumerate((25e3, 31e3, 37e3))]
-    kw = dict(det_size=5e-5, det_px=1e-6, det_py=1.0, z_detector=0.03, dtype=dtype, max_resident=1)
-    eager = run_batch(c, N, dx, ms, **kw)
-    for _ in range(3):
-        got = run_batch(c, N, dx, ms, use_graph=True, **kw)
-        c.synchronize()
-        assert np.array_equal(got, eager)
-        assert np.array_equal(run_batch(c, N, dx, ms, **kw), eager)
+    for max_resident in (1, 0):
+        kw = dict(det_size=5e-5, det_px=1e-6, det_py=1.0, z_detector=0.03, dtype=dtype, max_resident=max_resident)
+        eager = run_batch(c, N, dx, ms, **kw)
+        for _ in range(3):
+            got = run_batch(c, N, dx, ms, use_graph=True, **kw)
+            c.synchronize()
+            assert np.array_equal(got, eager)
+            assert np.array_equal(run_batch(c, N, dx, ms, **kw), eager)
     c.close()
 
 
