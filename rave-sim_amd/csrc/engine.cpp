@@ -373,6 +373,7 @@ Context::Context(int dev) : device(dev) {
     if (ch == ' ') ch = '_';
 }
 Context::~Context() {
+  graph_cache.reset();
   plans_.clear();
   for (Stage &sg : stage_)
     for (int b = 0; b < 2; ++b) {
@@ -628,9 +629,7 @@ template <typename S> struct BatchRun {
   bool needs_snap = false, needs_hist = false;
   int group = 1, lanes = 1, spans = 1;
   Plan *plan = nullptr;
-  DevBuf dbounds;
-  std::vector<DevBuf> dacc;
-  std::vector<std::vector<DevBuf>> fbuf, sbuf, hbuf;
+  RunBuffers buf;
   std::vector<DevBuf> sample_bufs;
   std::vector<std::unique_ptr<Exec<S>>> groups;
   MaskCache<S> masks;
@@ -638,15 +637,26 @@ template <typename S> struct BatchRun {
 
   BatchRun(Context &c, const RunConfig &r, std::vector<Member> &m) : ctx(c), rc(r), members(m) {}
 
-  // plan, buffers, compiled groups and the shared grating decisions, with
-  // the uploads on the context stream; `lanes_max` bounds the lanes (0: no
-  // bound beyond the default)
-  void setup(int lanes_max = 0) {
+  // the key under which this run's buffers and graph may be reused: what
+  // decides the launch sequence before any buffer exists
+  static std::string shape_key(const RunConfig &rc, const std::vector<Member> &members) {
+    std::ostringstream k;
+    k << rc.N << ' ' << rc.f64 << ' ' << rc.mode << ' ' << rc.n_out << ' ' << detector_pixels(rc.det_size, rc.det_px)
+      << ' ' << rc.max_resident << ' ' << rc.lanes << ' ' << rc.spans << ' ' << rc.lgR << ' ' << rc.lgA << ' '
+      << rc.tileA << ' ' << rc.tileB << ' ' << members.size() << ':';
+    for (const auto &m : members) {
+      for (const auto &o : m.ops) k << (char)('A' + (int)o.kind);
+      k << '/' << m.out_base << ',';
+    }
+    return k.str();
+  }
+
+  // plan, buffers (taken from `reuse` when they fit), compiled groups and
+  // the shared grating decisions, with the uploads on the context stream;
+  // `lanes_max` bounds the lanes (0: no bound beyond the default)
+  void setup(RunBuffers *reuse = nullptr, int lanes_max = 0) {
     npx = detector_pixels(rc.det_size, rc.det_px);
     const std::vector<int64_t> hb = detector_bounds(rc.N, rc.dx, rc.det_size, rc.det_px);
-    dbounds = DevBuf(hb.size() * sizeof(int64_t));
-    hip_check(hipMemcpyAsync(dbounds.p, hb.data(), hb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream),
-              "bounds upload");
     nacc = (size_t)rc.n_out * std::max<int64_t>(npx, 0);
     for (auto &m : members)
       for (auto &o : m.ops) {
@@ -675,28 +685,48 @@ template <typename S> struct BatchRun {
     if (lanes > 1 && rc.max_resident <= 0) group = std::max(1, group / lanes);
     plan = &ctx.plan(rc.N, rc.f64, rc.lgR, rc.lgA, (int)std::min<size_t>(members.size(), 65535), rc.tileA,
                      rc.tileB);
-
-    // per-lane accumulators (summed in lane order on the host)
-    for (int l = 0; l < lanes; ++l) {
-      dacc.emplace_back(std::max<size_t>(nacc, 1) * sizeof(double));
-      hip_check(launch_fill_f64((double *)dacc.back().p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
-    }
+    width = std::min(members.size(), (size_t)group);
 
     // Field (and snapshot) buffers of one group per lane, reused by every
     // group of the lane: each member program starts by generating or
-    // uploading its field.
-    fbuf.resize(lanes);
-    sbuf.resize(lanes);
-    hbuf.resize(lanes);
-    width = std::min(members.size(), (size_t)group);
-    for (int l = 0; l < lanes; ++l) {
-      if (!rc.fields)
-        for (size_t i = 0; i < width; ++i) fbuf[l].emplace_back(fbytes);
-      if (needs_snap)
-        for (size_t i = 0; i < width; ++i) sbuf[l].emplace_back(fbytes);
-      if (needs_hist)
-        for (size_t i = 0; i < width; ++i) hbuf[l].emplace_back(fbytes);
+    // uploading its field.  Per-lane accumulators (summed in lane order on
+    // the host).  Buffers of a previous run of this shape are taken over
+    // when they have the sizes this run needs.
+    auto fits = [&](const std::vector<std::vector<DevBuf>> &v, bool want) {
+      if ((int)v.size() != lanes) return false;
+      for (const auto &lb : v) {
+        if (lb.size() != (want ? width : 0)) return false;
+        for (const auto &b : lb)
+          if (b.bytes != fbytes) return false;
+      }
+      return true;
+    };
+    bool take = reuse && reuse->bounds.bytes == hb.size() * sizeof(int64_t) && (int)reuse->acc.size() == lanes &&
+                fits(reuse->field, !rc.fields) && fits(reuse->snap, needs_snap) && fits(reuse->hist, needs_hist);
+    if (take)
+      for (const auto &a : reuse->acc) take &= a.bytes == std::max<size_t>(nacc, 1) * sizeof(double);
+    if (take) {
+      buf = std::move(*reuse);
+    } else {
+      buf = RunBuffers{};
+      buf.bounds = DevBuf(hb.size() * sizeof(int64_t));
+      buf.field.resize(lanes);
+      buf.snap.resize(lanes);
+      buf.hist.resize(lanes);
+      for (int l = 0; l < lanes; ++l) {
+        buf.acc.emplace_back(std::max<size_t>(nacc, 1) * sizeof(double));
+        if (!rc.fields)
+          for (size_t i = 0; i < width; ++i) buf.field[l].emplace_back(fbytes);
+        if (needs_snap)
+          for (size_t i = 0; i < width; ++i) buf.snap[l].emplace_back(fbytes);
+        if (needs_hist)
+          for (size_t i = 0; i < width; ++i) buf.hist[l].emplace_back(fbytes);
+      }
     }
+    hip_check(hipMemcpyAsync(buf.bounds.p, hb.data(), hb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx.stream),
+              "bounds upload");
+    for (int l = 0; l < lanes; ++l)
+      hip_check(launch_fill_f64((double *)buf.acc[l].p, (int64_t)nacc, 0.0, ctx.stream), "acc fill");
     // sample elements: device copies of the members' pixel tables for this
     // run, pointed at by their SAMPLE masks
     auto upload_d = [&](const std::vector<double> &h) -> const double * {
@@ -749,12 +779,12 @@ template <typename S> struct BatchRun {
       ex->spans = spans;
       for (size_t i = g0; i < g1; ++i) {
         ex->mem.push_back(&members[i]);
-        ex->field.push_back(rc.fields ? rc.fields[i] : fbuf[lane][i - g0].p);
-        if (needs_snap) ex->snap.push_back(sbuf[lane][i - g0].p);
-        if (needs_hist) ex->hist.push_back(hbuf[lane][i - g0].p);
+        ex->field.push_back(rc.fields ? rc.fields[i] : buf.field[lane][i - g0].p);
+        if (needs_snap) ex->snap.push_back(buf.snap[lane][i - g0].p);
+        if (needs_hist) ex->hist.push_back(buf.hist[lane][i - g0].p);
       }
-      ex->acc = (double *)dacc[lane].p;
-      ex->bounds = (const int64_t *)dbounds.p;
+      ex->acc = (double *)buf.acc[lane].p;
+      ex->bounds = (const int64_t *)buf.bounds.p;
       ex->npx = npx;
       ex->compile();
       groups.push_back(std::move(ex));
@@ -776,6 +806,30 @@ template <typename S> struct BatchRun {
     }
   }
 
+  // everything the captured launches depend on besides the argument
+  // records' contents: launch records, streams, and the buffers and sizes
+  // passed to kernels by value
+  void launch_key(std::string &k) const {
+    auto put = [&](const void *p, size_t n) { k.append((const char *)p, n); };
+    for (auto &ex : groups) {
+      put(&ex->st, sizeof ex->st);
+      put(&ex->bounds, sizeof ex->bounds);
+      put(&ex->npx, sizeof ex->npx);
+      for (const auto &l : ex->launches) {
+        put(&l.kind, sizeof l.kind);
+        put(&l.lgL, sizeof l.lgL);
+        put(&l.row, sizeof l.row);
+        put(&l.xv, sizeof l.xv);
+        put(&l.geom, sizeof l.geom);
+        put(&l.arg_off, sizeof l.arg_off);
+        put(&l.nf, sizeof l.nf);
+        put(&l.dst, sizeof l.dst);
+        put(&l.src, sizeof l.src);
+        put(&l.n, sizeof l.n);
+      }
+    }
+  }
+
   // every group's launches on its lane, argument records at fa / da
   void enqueue(const FieldArgs *fa, const DetArgs *da) {
     size_t ofa = 0, oda = 0;
@@ -790,8 +844,9 @@ template <typename S> struct BatchRun {
   void download() {
     accs.assign(lanes, std::vector<double>(nacc));
     for (int l = 0; l < lanes && nacc; ++l)
-      hip_check(hipMemcpyAsync(accs[l].data(), dacc[l].p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
-                "acc download");
+      hip_check(
+          hipMemcpyAsync(accs[l].data(), buf.acc[l].p, nacc * sizeof(double), hipMemcpyDeviceToHost, ctx.stream),
+          "acc download");
   }
 
   // after the stream drained: pass timings and the scaled detector rows
@@ -813,12 +868,32 @@ template <typename S> struct BatchRun {
   }
 };
 
+}  // namespace
+
+// The last captured batch run of a context: its instantiated graph, the
+// argument buffers and run buffers its launches address, and the keys that
+// tell whether a new run would record the same launches.
+struct GraphCache {
+  std::string shape, launches;
+  hipGraphExec_t exec = nullptr;
+  DevBuf dfa, dda;
+  std::vector<RunBuffers> bufs;
+  ~GraphCache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+namespace {
+
 // Enqueue runs (already set up) on the context's lanes between one argument
 // upload and one synchronisation.  whole_graph: the launches of every lane
 // are captured as one hipGraph (the lanes fork from and join the context
-// stream through events inside the capture), instantiated and launched.
+// stream through events inside the capture) and instantiated; the graph and
+// the buffers it addresses stay in ctx.graph_cache, and a later run whose
+// launches are the same replays it with its own argument records.
 template <typename S>
-void execute_runs(Context &ctx, std::vector<std::unique_ptr<BatchRun<S>>> &runs, bool whole_graph) {
+void execute_runs(Context &ctx, std::vector<std::unique_ptr<BatchRun<S>>> &runs, bool whole_graph,
+                  const std::string &shape = std::string()) {
   std::vector<FieldArgs> hfa;
   std::vector<DetArgs> hda;
   int lanes = 1;
@@ -826,14 +901,30 @@ void execute_runs(Context &ctx, std::vector<std::unique_ptr<BatchRun<S>>> &runs,
     r->append_args(hfa, hda);
     lanes = std::max(lanes, r->lanes);
   }
-  DevBuf dfa(std::max<size_t>(hfa.size(), 1) * sizeof(FieldArgs));
-  DevBuf dda(std::max<size_t>(hda.size(), 1) * sizeof(DetArgs));
+  const size_t bfa = std::max<size_t>(hfa.size(), 1) * sizeof(FieldArgs);
+  const size_t bda = std::max<size_t>(hda.size(), 1) * sizeof(DetArgs);
+  GraphCache *gc = whole_graph ? ctx.graph_cache.get() : nullptr;
+  DevBuf dfa, dda;
+  if (gc && gc->dfa.bytes == bfa && gc->dda.bytes == bda) {
+    dfa = std::move(gc->dfa);
+    dda = std::move(gc->dda);
+  } else {
+    dfa = DevBuf(bfa);
+    dda = DevBuf(bda);
+  }
   if (!hfa.empty())
     hip_check(hipMemcpyAsync(dfa.p, hfa.data(), hfa.size() * sizeof(FieldArgs), hipMemcpyHostToDevice, ctx.stream),
               "args upload");
   if (!hda.empty())
     hip_check(hipMemcpyAsync(dda.p, hda.data(), hda.size() * sizeof(DetArgs), hipMemcpyHostToDevice, ctx.stream),
               "det args upload");
+  std::string lkey;
+  if (whole_graph) {
+    lkey.append((const char *)&dfa.p, sizeof dfa.p);
+    lkey.append((const char *)&dda.p, sizeof dda.p);
+    for (auto &r : runs) r->launch_key(lkey);
+  }
+  const bool replay = gc && gc->exec && gc->launches == lkey;
   std::vector<hipEvent_t> evs;
   auto new_event = [&]() {
     hipEvent_t e = nullptr;
@@ -847,72 +938,102 @@ void execute_runs(Context &ctx, std::vector<std::unique_ptr<BatchRun<S>>> &runs,
       for (hipEvent_t x : e) (void)hipEventDestroy(x);
     }
   } events_free{evs};
-  hipGraph_t graph = nullptr;
-  if (whole_graph) hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-  try {
-    // lanes start after the uploads on the context stream and join it at the end
-    if (lanes > 1) {
-      hipEvent_t ready = new_event();
-      hip_check(hipEventRecord(ready, ctx.stream), "event record");
-      for (int l = 1; l < lanes; ++l) hip_check(hipStreamWaitEvent(ctx.lane_stream(l), ready, 0), "stream wait");
-    }
-    size_t ofa = 0, oda = 0;
-    for (auto &r : runs) {
-      r->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
-      for (auto &ex : r->groups) {
-        ofa += ex->fargs.size();
-        oda += ex->dargs.size();
+  hipGraphExec_t exec = replay ? gc->exec : nullptr;
+  if (!replay) {
+    hipGraph_t graph = nullptr;
+    if (whole_graph) hip_check(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+      // lanes start after the uploads on the context stream and join it at the end
+      if (lanes > 1) {
+        hipEvent_t ready = new_event();
+        hip_check(hipEventRecord(ready, ctx.stream), "event record");
+        for (int l = 1; l < lanes; ++l) hip_check(hipStreamWaitEvent(ctx.lane_stream(l), ready, 0), "stream wait");
       }
+      size_t ofa = 0, oda = 0;
+      for (auto &r : runs) {
+        r->enqueue((const FieldArgs *)dfa.p + ofa, (const DetArgs *)dda.p + oda);
+        for (auto &ex : r->groups) {
+          ofa += ex->fargs.size();
+          oda += ex->dargs.size();
+        }
+      }
+      for (int l = 1; l < lanes; ++l) {
+        hipEvent_t done = new_event();
+        hip_check(hipEventRecord(done, ctx.lane_stream(l)), "event record");
+        hip_check(hipStreamWaitEvent(ctx.stream, done, 0), "stream wait");
+      }
+    } catch (...) {
+      if (whole_graph) {
+        (void)hipStreamEndCapture(ctx.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+      }
+      throw;
     }
-    for (int l = 1; l < lanes; ++l) {
-      hipEvent_t done = new_event();
-      hip_check(hipEventRecord(done, ctx.lane_stream(l)), "event record");
-      hip_check(hipStreamWaitEvent(ctx.stream, done, 0), "stream wait");
-    }
-  } catch (...) {
     if (whole_graph) {
-      (void)hipStreamEndCapture(ctx.stream, &graph);
-      if (graph) (void)hipGraphDestroy(graph);
+      hip_check(hipStreamEndCapture(ctx.stream, &graph), "end capture");
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hip_check(ie, "graph instantiate");
     }
-    throw;
   }
   if (whole_graph) {
-    hip_check(hipStreamEndCapture(ctx.stream, &graph), "end capture");
-    hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    hip_check(ie, "graph instantiate");
-    const hipError_t le = hipGraphLaunch(ex, ctx.stream);
-    if (le == hipSuccess) hip_check(hipStreamSynchronize(ctx.stream), "graph sync");
-    (void)hipGraphExecDestroy(ex);
-    hip_check(le, "graph launch");
+    // the cache owns the graph and buffers from here on, replayed or new
+    if (!ctx.graph_cache) ctx.graph_cache = std::make_unique<GraphCache>();
+    gc = ctx.graph_cache.get();
+    if (!replay) {
+      if (gc->exec) (void)hipGraphExecDestroy(gc->exec);
+      gc->exec = exec;
+      gc->launches = lkey;
+    }
+    hip_check(hipGraphLaunch(gc->exec, ctx.stream), "graph launch");
   }
   for (auto &r : runs) r->download();
   hip_check(hipStreamSynchronize(ctx.stream), "sync");
+  if (whole_graph) {
+    gc->dfa = std::move(dfa);
+    gc->dda = std::move(dda);
+    gc->bufs.clear();
+    for (auto &r : runs) gc->bufs.push_back(std::move(r->buf));
+    gc->shape = shape;
+  }
+}
+
+// buffers of the cached graph run for the i-th of n runs of this shape
+inline RunBuffers *cached_buffers(Context &ctx, const std::string &shape, size_t i, size_t n) {
+  GraphCache *gc = ctx.graph_cache.get();
+  if (!gc || gc->shape != shape || gc->bufs.size() != n) return nullptr;
+  return &gc->bufs[i];
 }
 
 template <typename S>
 std::vector<double> run_members_t(Context &ctx, const RunConfig &rc, std::vector<Member> &members) {
   hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  const bool graph = rc.use_graph && rc.whole_graph && !rc.fields;
+  const std::string shape = graph ? "1|" + BatchRun<S>::shape_key(rc, members) : std::string();
   std::vector<std::unique_ptr<BatchRun<S>>> runs;
   runs.push_back(std::make_unique<BatchRun<S>>(ctx, rc, members));
-  runs[0]->setup();
-  execute_runs<S>(ctx, runs, rc.use_graph && rc.whole_graph);
+  runs[0]->setup(graph ? cached_buffers(ctx, shape, 0, 1) : nullptr);
+  execute_runs<S>(ctx, runs, graph, shape);
   return runs[0]->finish();
 }
 
 template <typename S>
 std::vector<std::vector<double>> run_mixed_t(Context &ctx, const std::vector<MixedPart> &parts, bool whole_graph) {
   hip_check(hipSetDevice(ctx.device), "hipSetDevice");
+  std::string shape;
+  if (whole_graph) {
+    shape = std::to_string(parts.size()) + "|";
+    for (const MixedPart &p : parts) shape += BatchRun<S>::shape_key(p.rc, *p.members) + ";";
+  }
   std::vector<std::unique_ptr<BatchRun<S>>> runs;
-  for (const MixedPart &p : parts) {
-    RunConfig rc = p.rc;
+  for (size_t i = 0; i < parts.size(); ++i) {
+    RunConfig rc = parts[i].rc;
     rc.use_graph = whole_graph;
     rc.whole_graph = whole_graph;
-    runs.push_back(std::make_unique<BatchRun<S>>(ctx, rc, *p.members));
-    runs.back()->setup();
+    runs.push_back(std::make_unique<BatchRun<S>>(ctx, rc, *parts[i].members));
+    runs.back()->setup(whole_graph ? cached_buffers(ctx, shape, i, parts.size()) : nullptr);
   }
-  execute_runs<S>(ctx, runs, whole_graph);
+  execute_runs<S>(ctx, runs, whole_graph, shape);
   std::vector<std::vector<double>> out;
   for (auto &r : runs) out.push_back(r->finish());
   return out;

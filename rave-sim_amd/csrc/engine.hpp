@@ -168,6 +168,7 @@ struct RunConfig {
 };
 
 class Context;
+struct GraphCache;
 
 // Run all members; returns scaled detector rows, n_out x n_px (row-major).
 std::vector<double> run_members(Context &ctx, const RunConfig &rc, std::vector<Member> &members);
@@ -232,6 +233,10 @@ class Context {
   std::string device_key;
   std::vector<Tuned> tuned;
   const Tuned *find_tuned(int lgN, bool f64, int members = 1) const;
+  // the instantiated graph of the last captured batch run and the device
+  // buffers its launches address: a run of the same shape replays it with
+  // new argument records instead of capturing again
+  std::unique_ptr<GraphCache> graph_cache;
   // Copy N natural-order samples of a .npy vector (member m's upload_npy)
   // to the device field dst on stream st.  Up to kStageThreads host threads
   // each stream one contiguous part: chunks are read into the thread's two
@@ -306,6 +311,15 @@ struct DevBuf {
   DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
   DevBuf &operator=(DevBuf &&o) noexcept;
   ~DevBuf();
+};
+
+// Device buffers of one lockstep run: pixel bounds, the per-lane FP64
+// accumulators, and the field / snapshot / history buffers of one group per
+// lane.
+struct RunBuffers {
+  DevBuf bounds;
+  std::vector<DevBuf> acc;
+  std::vector<std::vector<DevBuf>> field, snap, hist;
 };
 
 }  // namespace rs

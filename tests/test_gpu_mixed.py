@@ -99,6 +99,35 @@ def test_mixed_graph_replays_match_eager(dtype):
     c.close()
 
 
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_graph_replay_takes_new_parameters(dtype):
+    """A second graph run of the same shape replays the cached graph with its
+    own argument records: other energies, pitches, duty cycles and source
+    positions give that run's eager rows, not the first run's; a run of
+    another shape captures anew."""
+    import ravesim
+    from ravesim.core import run_mixed
+    a, _ = mixed_case(21, n_sims=5, lg=(13, 16))
+    b, _ = mixed_case(21, n_sims=5, lg=(13, 16))
+    rng = np.random.default_rng(5)
+    for s in b:  # same structure, new physics
+        for m in s.members:
+            m.wl *= float(rng.uniform(0.8, 1.2))
+            m.source_x += 3e-8
+            g = m.gratings[0]
+            m.gratings = [ravesim.Grating(pitch=g.pitch * 1.1, dc=(0.47, 0.47), z_start=g.z_start,
+                                          thickness=g.thickness, nr_steps=g.nr_steps, db_a=g.db_a * 1.3)]
+    other, _ = mixed_case(22, n_sims=4, lg=(12, 14))
+    c = ravesim.Context(0)
+    ea, eb, eo = (run_mixed(c, x, dtype=dtype) for x in (a, b, other))
+    for sims, ref in ((a, ea), (b, eb), (a, ea), (other, eo), (b, eb)):
+        got = run_mixed(c, sims, dtype=dtype, use_graph=True)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r)
+    assert not all(np.array_equal(x, y) for x, y in zip(ea, eb))
+    c.close()
+
+
 def test_mixed_batch_errors():
     import ravesim
     from ravesim._lib import RsError
