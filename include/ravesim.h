@@ -383,7 +383,9 @@ typedef struct rs_pass_stat {
  * lanes), by wall clock, and keep the fastest for this context and that batch
  * size; runs of other batch sizes use the entry tuned for the nearest one.
  * out4 (optional) = {groups, lg top, lg middle, lg row}; ms_per_slice
- * (optional) = measured time of one slice of one field with it. */
+ * (optional) = measured time of one slice of one field with it (the
+ * difference of a 4-slice and a 16-slice run, per slice and field, so that
+ * what a run pays once is left out). */
 int rs_autotune(rs_ctx *ctx, int64_t N, int dtype, int members, int32_t *out4, double *ms_per_slice);
 /* Fix the digit-group split for (N, dtype) (lgR = log2 row length, lgA = log2
  * top column length, -1 = default); persisted tuning results are re-applied

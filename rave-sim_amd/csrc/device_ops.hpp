@@ -625,48 +625,94 @@ __device__ __forceinline__ void apply_inter_twiddles(const Geom &g, int q, int64
   }
 }
 
+// A table of N entries copied from global memory into LDS by the NT threads
+// of a workgroup in two halves, load() into registers and store() from them,
+// so that the reads of several tables are in flight together
+template <typename T, int N, int NT>
+struct LdsStage {
+  static constexpr int K = N > 0 ? (N + NT - 1) / NT : 0;
+  T r[K > 0 ? K : 1];
+  __device__ __forceinline__ void load(const T *__restrict__ src, int t) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = t + k * NT;
+      if (N % NT == 0 || i < N) r[k] = src[i];
+    }
+  }
+  __device__ __forceinline__ void store(T *__restrict__ dst, int t) const {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = t + k * NT;
+      if (N % NT == 0 || i < N) dst[i] = r[k];
+    }
+  }
+};
+
 // Column powers of a column-pass tile, shared through LDS: the TPF threads of
 // column c need W_M^{m TPF col} (col = c0 + c, m < 16) alike, so the
 // workgroup forms each of the 16 C values once, P[m * C + c], one split-table
 // read each.  A thread's twiddles are then tw[m] = W_M^{q col} * P[m][c]:
 // one split-table read and 15 products (three roundings from the exact
 // value, against four or five for the power recurrence above).
-template <typename T2, int L, int C>
+// BATCH: every table read of the thread first, then the products; else one
+// entry at a time.
+template <bool BATCH, typename T2, int L, int C>
 __device__ __forceinline__ void build_col_powers(const Geom &g, int64_t c0, T2 *__restrict__ P, int t) {
   const T2 *hi = (const T2 *)g.tw_hi;
   const T2 *lo = (const T2 *)g.tw_lo;
   const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
   constexpr int TPF = L / 16;
   constexpr int NT = C * TPF;
+  constexpr int K = (16 * C + NT - 1) / NT;
+  const uint32_t lm = (1u << g.tw_sh) - 1;
+  if constexpr (!BATCH) {
 #pragma unroll
-  for (int i0 = 0; i0 < 16 * C; i0 += NT) {
-    const int i = i0 + t;
+    for (int k = 0; k < K; ++k) {
+      const int i = k * NT + t;
+      if (16 * C % NT == 0 || i < 16 * C) {
+        const uint32_t m = (uint32_t)(i / C), cl = (uint32_t)c0 + (uint32_t)(i % C);
+        P[i] = split_twiddle(hi, lo, g.tw_sh, (m * (uint32_t)TPF * cl) & mask);
+      }
+    }
+    return;
+  }
+  T2 h[K], l[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = k * NT + t;
     if (16 * C % NT == 0 || i < 16 * C) {
       const uint32_t m = (uint32_t)(i / C), cl = (uint32_t)c0 + (uint32_t)(i % C);
-      P[i] = split_twiddle(hi, lo, g.tw_sh, (m * (uint32_t)TPF * cl) & mask);
+      const uint32_t e = (m * (uint32_t)TPF * cl) & mask;
+      h[k] = hi[e >> g.tw_sh];
+      l[k] = lo[e & lm];
     }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = k * NT + t;
+    if (16 * C % NT == 0 || i < 16 * C) P[i] = cmul(h[k], l[k]);
   }
 }
 
-template <typename T2, int L, int C>
-__device__ __forceinline__ void col_twiddles(const Geom &g, int q, int c, int64_t col, const T2 *__restrict__ P,
-                                             T2 (&tw)[16]) {
+// W_M^{q col}: a thread's own factor, read once before the powers are formed
+// so that its latency overlaps theirs
+template <typename T2, int L>
+__device__ __forceinline__ T2 col_base_twiddle(const Geom &g, int q, int64_t col) {
   const T2 *hi = (const T2 *)g.tw_hi;
   const T2 *lo = (const T2 *)g.tw_lo;
   const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
-  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * (uint32_t)col) & mask);
+  return split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * (uint32_t)col) & mask);
+}
+
+template <typename T2, int C>
+__device__ __forceinline__ void col_twiddles(T2 wq, int c, const T2 *__restrict__ P, T2 (&tw)[16]) {
   tw[0] = wq;
 #pragma unroll
   for (int m = 1; m < 16; ++m) tw[m] = cmul(wq, P[m * C + c]);
 }
 
-template <bool CONJ, typename T2, int L, int C>
-__device__ __forceinline__ void apply_col_twiddles(const Geom &g, int q, int c, int64_t col,
-                                                   const T2 *__restrict__ P, T2 (&v)[16]) {
-  const T2 *hi = (const T2 *)g.tw_hi;
-  const T2 *lo = (const T2 *)g.tw_lo;
-  const uint32_t mask = (uint32_t)((int64_t)L * g.S - 1);
-  const T2 wq = split_twiddle(hi, lo, g.tw_sh, ((uint32_t)q * (uint32_t)col) & mask);
+template <bool CONJ, typename T2, int C>
+__device__ __forceinline__ void apply_col_twiddles(T2 wq, int c, const T2 *__restrict__ P, T2 (&v)[16]) {
   v[0] = CONJ ? cmulc(v[0], wq) : cmul(v[0], wq);
 #pragma unroll
   for (int m = 1; m < 16; ++m) {

@@ -289,7 +289,8 @@ class Context {
 // column passes, on a synthetic grating slice loop (mask + propagation) run
 // as a batch of `members` fields runs by default (grouping and lanes of
 // run_members), by wall clock, and record the fastest in ctx.tuned for that
-// batch size.  Returns it.
+// batch size.  The figure is the marginal cost of a slice: the difference of
+// runs of reps + 1 and 4 (reps + 1) slices, per slice and field.  Returns it.
 Context::Tuned autotune(Context &ctx, int64_t N, bool f64, int reps = 3, int members = 1);
 
 // Tuning tables as text, one entry per line:

@@ -84,19 +84,8 @@ __global__ void __launch_bounds__(C *(L / 16))
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
   const int t = threadIdx.x;
-  // twiddle tables of the small Stockham stages, staged in LDS (first read
-  // after the first stage's barrier)
   constexpr int NSTAB = small_tab_entries(L);
   __shared__ T2 stab[NSTAB > 0 ? NSTAB : 1];
-  if constexpr (NSTAB > 0)
-    for (int i = t; i < NSTAB; i += C * TPF) stab[i] = twf[L + i];
-  if constexpr (KTAB) {
-    if (A.k.on) {
-      const float2 *gt = (const float2 *)g.cis_tab;
-      for (int i = t; i < kCisTab; i += C * TPF) ktab[i] = gt[i];
-      __syncthreads();
-    }
-  }
   const int64_t tile = g.tile0 + blockIdx.x;
   const TileMap<L, C, ROW> tm(g, tile, t);
   const int c = tm.c, q = tm.q;
@@ -112,10 +101,28 @@ __global__ void __launch_bounds__(C *(L / 16))
   if constexpr (XV == XV_GEN) gen = xop && (A.x.kind == XOP_POINT || A.x.kind == XOP_ONES || A.x.from_ones);
 
   const int64_t nb = tm.nb, c0 = tm.c0;
-  // column powers first: their table reads go out before the field loads
-  if constexpr (CPOW) {
-    build_col_powers<T2, L, C>(g, c0, cpow, t);
-    __syncthreads();
+  // LDS tables: the twiddles of the small Stockham stages (first read after
+  // the first stage's barrier), the spectrum phase table and the column
+  // powers.
+  // complex64 passes issue the field loads first and then every table read
+  // before the first LDS write; the table reads wait for the field loads
+  // with their own, so such a pass pays one memory latency before its first
+  // DFT instead of one per table (MI355X, 2^26: row 4096 104 -> 100 us,
+  // top 64 228 -> 216 us per launch).  Column passes of 128 keep the tables
+  // first (field loads early measured 217 -> 244 us there), and so do the
+  // complex128 passes, which have no registers to spare for table entries
+  // next to 64 in flight (their staging arrays went to scratch memory).
+  constexpr bool EARLY = sizeof(S) == 4 && (ROW || L != 128);
+  const bool kt = KTAB && A.k.on;
+  if constexpr (!EARLY) {
+    if constexpr (NSTAB > 0)
+      for (int i = t; i < NSTAB; i += C * TPF) stab[i] = twf[L + i];
+    if (kt) {
+      const float2 *gt = (const float2 *)g.cis_tab;
+      for (int i = t; i < kCisTab; i += C * TPF) ktab[i] = gt[i];
+    }
+    if constexpr (CPOW) build_col_powers<false, T2, L, C>(g, c0, cpow, t);
+    if (CPOW || kt) __syncthreads();
   }
   // cached grating decisions of this thread, loaded with the field (their
   // first use comes after the inverse DFT's barriers)
@@ -131,6 +138,18 @@ __global__ void __launch_bounds__(C *(L / 16))
 #pragma unroll
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
+  T2 wq;
+  if constexpr (CPOW) wq = col_base_twiddle<T2, L>(g, q, c0 + c);
+  if constexpr (EARLY) {
+    LdsStage<T2, NSTAB, C * TPF> st;
+    st.load(twf + L, t);
+    LdsStage<float2, KTAB ? kCisTab : 0, C * TPF> sk;
+    if (kt) sk.load((const float2 *)g.cis_tab, t);
+    if constexpr (CPOW) build_col_powers<true, T2, L, C>(g, c0, cpow, t);
+    st.store(stab, t);
+    if (kt) sk.store(ktab, t);
+    if (CPOW || kt) __syncthreads();
+  }
   {
     const int64_t col = c0 + c;
     const int64_t rr = tile * C + c;
@@ -142,13 +161,13 @@ __global__ void __launch_bounds__(C *(L / 16))
     bool have_tw = false;
     if (!ROW && A.do_inv) {
       if constexpr (KEEP_TW) {
-        if constexpr (CPOW) col_twiddles<T2, L, C>(g, q, c, col, cpow, tw);
+        if constexpr (CPOW) col_twiddles<T2, C>(wq, c, cpow, tw);
         else inter_twiddles<T2, L>(g, q, col, tw);
         have_tw = true;
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
       } else {
-        if constexpr (CPOW) apply_col_twiddles<true, T2, L, C>(g, q, c, col, cpow, v);
+        if constexpr (CPOW) apply_col_twiddles<true, T2, C>(wq, c, cpow, v);
         else apply_inter_twiddles<true, T2, L>(g, q, col, v);
       }
       stockham<T2, L, 1, true>(v, fib, q, twf, stab);
@@ -190,7 +209,7 @@ __global__ void __launch_bounds__(C *(L / 16))
 #pragma unroll
           for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], tw[m]);
         } else if constexpr (CPOW) {
-          apply_col_twiddles<false, T2, L, C>(g, q, c, col, cpow, v);
+          apply_col_twiddles<false, T2, C>(wq, c, cpow, v);
         } else {
           apply_inter_twiddles<false, T2, L>(g, q, col, v);
         }

@@ -98,7 +98,7 @@ constexpr int fiber_tab_entries(int L) { return L + small_tab_entries(L); }
 // as many fibers at L = 128 and 512 (and complex128 at 64), half as many
 // elsewhere (64 KB wide tiles, two workgroups per CU).
 constexpr int col_fibers(int L, bool f64, int tile = 0) {
-  int c;
+  int c = 0;
   if (L >= 1024) c = (f64 ? 8192 : 16384) / L;
   else if (L == 512) c = f64 ? 8 : 16;
   else if (L <= 16) c = 64;
