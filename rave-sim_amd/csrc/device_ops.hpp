@@ -213,7 +213,22 @@ __device__ __forceinline__ T2 split_twiddle(const T2 *__restrict__ hi, const T2 
   return cmul(hi[e >> sh], lo[e & ((1u << sh) - 1)]);
 }
 
-template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV>
+// Exchange ordering between the Stockham stages of a fiber.  WAVE: every
+// thread of the fiber sits in one wavefront (rows of up to 1024, column tiles
+// of one wavefront), whose LDS operations execute in program order, so the
+// exchange needs no workgroup barrier, only a fence that keeps the compiler
+// from moving LDS accesses across it.
+template <bool WAVE> __device__ __forceinline__ void fiber_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV, bool WAVE>
 __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
                                                const T2 *__restrict__ twf, const T2 *stab) {
   constexpr int TPF = L / 16;
@@ -257,7 +272,7 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i + r * NB] = x[i][r];
   } else {
-    __syncthreads();
+    fiber_sync<WAVE>();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       const int b = q + i * TPF;
@@ -267,7 +282,7 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
 #pragma unroll
       for (int r = 0; r < R; ++r) dst[lds_step<NS>(r)] = x[i][r];
     }
-    __syncthreads();
+    fiber_sync<WAVE>();
   }
 }
 
@@ -281,14 +296,14 @@ template <int L, int NS> constexpr int stage_radix() {
   return r;
 }
 
-template <typename T2, int L, int NS, bool INV>
+template <typename T2, int L, int NS, bool INV, bool WAVE = false>
 __device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf,
                                          const T2 *stab) {
   constexpr int R = stage_radix<L, NS>();
   constexpr bool FIRST = NS == 1;
   constexpr bool LAST = NS * R == L;
-  stockham_stage<T2, L, R, NS, FIRST, LAST, INV>(v, fib, q, twf, stab);
-  if constexpr (!LAST) stockham<T2, L, NS * R, INV>(v, fib, q, twf, stab);
+  stockham_stage<T2, L, R, NS, FIRST, LAST, INV, WAVE>(v, fib, q, twf, stab);
+  if constexpr (!LAST) stockham<T2, L, NS * R, INV, WAVE>(v, fib, q, twf, stab);
 }
 
 // ---------------------------------------------------------------------------

@@ -81,6 +81,13 @@ __global__ void __launch_bounds__(C *(L / 16))
   constexpr bool CPOW = !ROW && TPF >= 4 && kLdsPerCu / kWith == kLdsPerCu / kBase;
   __shared__ T2 cpow[CPOW ? 16 * C : 1];
 
+  // every thread of a fiber in one wavefront: rows of up to 1024 (a row's
+  // TPF threads are consecutive), column tiles of one wavefront
+  // (column passes keep their fibers across wavefronts: lanes of a wave take
+  // consecutive columns, for whole row segments per load; wave-local columns,
+  // 64 / TPF per wave, made the 2^26 top pass slower, 216 -> 347 us at 64)
+  constexpr bool WAVE = ROW ? TPF <= 64 : C * TPF <= 64;
+
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
   const int t = threadIdx.x;
@@ -102,8 +109,9 @@ __global__ void __launch_bounds__(C *(L / 16))
 
   const int64_t nb = tm.nb, c0 = tm.c0;
   // LDS tables: the twiddles of the small Stockham stages (first read after
-  // the first stage's barrier), the spectrum phase table and the column
-  // powers.
+  // the first exchange, so a workgroup barrier covers them only where the
+  // exchanges are wave-synchronous), the spectrum phase table and the
+  // column powers.
   // complex64 passes issue the field loads first and then every table read
   // before the first LDS write; the table reads wait for the field loads
   // with their own, so such a pass pays one memory latency before its first
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       for (int i = t; i < kCisTab; i += C * TPF) ktab[i] = gt[i];
     }
     if constexpr (CPOW) build_col_powers<false, T2, L, C>(g, c0, cpow, t);
-    if (CPOW || kt) __syncthreads();
+    if (CPOW || kt || (WAVE && NSTAB > 0)) __syncthreads();
   }
   // cached grating decisions of this thread, loaded with the field (their
   // first use comes after the inverse DFT's barriers)
@@ -148,7 +156,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     if constexpr (CPOW) build_col_powers<true, T2, L, C>(g, c0, cpow, t);
     st.store(stab, t);
     if (kt) sk.store(ktab, t);
-    if (CPOW || kt) __syncthreads();
+    if (CPOW || kt || (WAVE && NSTAB > 0)) __syncthreads();
   }
   {
     const int64_t col = c0 + c;
@@ -170,7 +178,7 @@ __global__ void __launch_bounds__(C *(L / 16))
         if constexpr (CPOW) apply_col_twiddles<true, T2, C>(wq, c, cpow, v);
         else apply_inter_twiddles<true, T2, L>(g, q, col, v);
       }
-      stockham<T2, L, 1, true>(v, fib, q, twf, stab);
+      stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
     }
 
     if constexpr (XV == XV_MASK) {
@@ -203,7 +211,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     }
 
     if (A.do_fwd) {
-      stockham<T2, L, 1, false>(v, fib, q, twf, stab);
+      stockham<T2, L, 1, false, WAVE>(v, fib, q, twf, stab);
       if (!ROW) {
         if (have_tw) {
 #pragma unroll
@@ -240,7 +248,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       }
     }
 
-    if (ROW && A.do_inv) stockham<T2, L, 1, true>(v, fib, q, twf, stab);
+    if (ROW && A.do_inv) stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
 
     {
       const gptr<nvec2<S>> p0 = data + nb;
