@@ -82,9 +82,9 @@ def test_polychromatic_batch_at_64_x_2_24(ctx):
 
 
 def test_served_request_at_2_28_matches_plain_c128():
-    """configs[4]'s second-largest size: a served phase-stepping sweep (8 G2
-    positions, 650 steps) at N = 2^28 in complex64 against the same chain run
-    position by position in the plain complex128 form."""
+    """A served phase-stepping sweep at N = 2^28, the second-largest served
+    grid size (8 G2 positions, 650 steps), in complex64 against the same
+    chain run position by position in the plain complex128 form."""
     import ravesim
     from ravesim.core import BatchMember, run_batch
     from ravesim.serve import Server, phase_sweep
@@ -123,9 +123,9 @@ def test_talbot_lau_with_sphere_matches_oracle(ctx, tmp_path, dtype, tol):
 
 
 def test_talbot_lau_with_sphere_at_2_26(ctx, tmp_path):
-    """configs[1] with its optional phantom at full size: 3 x 64 slices, 2
-    hops and 100 sphere rows (1 um z pixels), complex64 against the plain
-    complex128 form."""
+    """The 2^26 Talbot-Lau chain with its optional sphere phantom at full
+    size: 3 x 64 slices, 2 hops and 100 sphere rows (1 um z pixels),
+    complex64 against the plain complex128 form."""
     from ravesim.multisim import run_single_simulation
     from ravesim.simdir import write_simdir
     sd = write_simdir(tmp_path / "sim", *simcases.talbot_lau_phantom(tmp_path, N=2 ** 26, dtype="c8"))
