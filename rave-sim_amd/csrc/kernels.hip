@@ -73,6 +73,19 @@ __global__ void __launch_bounds__(256) kop_kernel(const FieldArgs *__restrict__ 
 // detector
 // ---------------------------------------------------------------------------
 
+// Sum of a 256-thread workgroup's values in a fixed order: each wavefront
+// reduces its 64 lanes with cross-lane xor exchanges (ds_bpermute, no LDS
+// storage, no barrier), then thread 0 adds the four wavefront sums, so the
+// order never depends on scheduling.  Valid in thread 0; red4 holds four
+// doubles and may be reused after the next barrier.
+__device__ __forceinline__ double block_sum256(double s, double *red4) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = s;
+  __syncthreads();
+  return (red4[0] + red4[1]) + (red4[2] + red4[3]);
+}
+
 // One workgroup per pixel; the members of the launch are visited in order
 // inside the workgroup, so members that share an accumulator row add into it
 // without races and in a fixed order (results do not depend on scheduling).
@@ -80,7 +93,7 @@ template <typename S>
 __global__ void __launch_bounds__(256) detect_kernel(const DetArgs *__restrict__ args, int nf,
                                                      const int64_t *__restrict__ bounds) {
   using T2 = typename C2T<S>::t;
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int k = blockIdx.x;
   const int64_t lo = bounds[k], hi = bounds[k + 1];
   for (int f = 0; f < nf; ++f) {
@@ -91,16 +104,10 @@ __global__ void __launch_bounds__(256) detect_kernel(const DetArgs *__restrict__
       const T2 z = u[j];
       s += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
     }
-    red[threadIdx.x] = s;
-    __syncthreads();
-#pragma unroll
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
+    const double tot = block_sum256(s, red);
     if (threadIdx.x == 0) {
       const double prev = A.accumulate ? A.acc[k] : 0.0;
-      A.acc[k] = prev + A.weight * red[0];
+      A.acc[k] = prev + A.weight * tot;
     }
     __syncthreads();
   }
@@ -148,7 +155,7 @@ template <typename S>
 __global__ void __launch_bounds__(256) detect_one_kernel(const void *__restrict__ data, double *__restrict__ acc,
                                                          const int64_t *__restrict__ bounds) {
   using T2 = typename C2T<S>::t;
-  __shared__ double red[256];
+  __shared__ double red[4];
   const int k = blockIdx.x;
   const int64_t lo = bounds[k], hi = bounds[k + 1];
   const T2 *u = (const T2 *)data;
@@ -157,14 +164,8 @@ __global__ void __launch_bounds__(256) detect_one_kernel(const void *__restrict_
     const T2 z = u[j];
     s += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
   }
-  red[threadIdx.x] = s;
-  __syncthreads();
-#pragma unroll
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) acc[k] = 0.0 + 1.0 * red[0];
+  const double tot = block_sum256(s, red);
+  if (threadIdx.x == 0) acc[k] = 0.0 + 1.0 * tot;
 }
 
 template <typename S>
