@@ -42,4 +42,9 @@ def test_snapshot_golden_vector(golden_dir):
         max(abs(ms["x_range"][0]), abs(ms["x_range"][1])))
     rows = bw.run_simulation(p, bw.PointSource(x=xs[0], z=ms["z"]), els, angles)
     assert len(rows) == 1
-    assert np.allclose(rows[0], np.array(d["known_data"]), rtol=0, atol=1e-14)
+    exp = np.array(d["known_data"])
+    # the reference's own bar (atol 1e-14 on values of 2.6e-9..2.2e-7), and a
+    # relative one: the fixture holds 9 significant digits and was made with
+    # the reference's own delta/beta, so it pins to ~1e-8 (measured 8.7e-9)
+    assert np.allclose(rows[0], exp, rtol=0, atol=1e-14)
+    assert np.linalg.norm(rows[0] - exp) / np.linalg.norm(exp) < 5e-8
