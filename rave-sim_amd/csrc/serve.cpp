@@ -25,6 +25,11 @@ constexpr int kSlotsPerLane = 3;
 // one, at most kMaxPassesPerTick): about the same device time per tick
 constexpr int64_t kPaceSamples = int64_t(1) << 26;
 constexpr int kMaxPassesPerTick = 64;
+// lanes of larger grids wait while a lane of a smaller grid has passes to
+// issue (shortest jobs first: a 2^26 request does not share the device with
+// a 2^30 one), except for one tick per kDeferTickS seconds, so that they
+// keep moving under a steady stream of small requests
+constexpr double kDeferTickS = 1.0;
 
 }  // namespace
 
@@ -139,7 +144,8 @@ struct Server::Lane {
   hipStream_t st = nullptr;
   std::vector<std::unique_ptr<Slot>> slots;
   size_t next = 0;
-  size_t bytes = 0;  // resident bytes of its jobs
+  size_t bytes = 0;          // resident bytes of its jobs
+  double deferred_at = 0.0;  // when it last started waiting for a smaller grid's lane (0: not waiting)
 };
 
 Server::Server(Context &ctx, double reserve_bytes, int max_jobs)
@@ -406,6 +412,14 @@ bool Server::tick() {
     size_t off = 0;
   };
   bool issued = false;
+  // the largest grid with passes to issue in each lane, and the smallest of those
+  std::vector<int64_t> lane_n(lanes_.size(), 0);
+  for (auto &jp : active_)
+    if (jp->cursor < jp->prog.size()) lane_n[(size_t)jp->lane] = std::max(lane_n[(size_t)jp->lane], jp->N);
+  int64_t min_n = 0;
+  for (int64_t n : lane_n)
+    if (n > 0 && (min_n == 0 || n < min_n)) min_n = n;
+  const double now = now_s();
   for (size_t li = 0; li < lanes_.size(); ++li) {
     Lane &ln = *lanes_[li];
     std::vector<Job *> js;
@@ -415,12 +429,24 @@ bool Server::tick() {
         js.push_back(jp.get());
         reps = std::max(reps, jp->pace);
       }
-    if (js.empty()) continue;
+    if (js.empty()) {
+      ln.deferred_at = 0.0;
+      continue;
+    }
+    bool deferred_tick = false;
+    if (lane_n[li] > min_n) {
+      if (ln.deferred_at == 0.0) ln.deferred_at = now;
+      if (now - ln.deferred_at < kDeferTickS) continue;
+      deferred_tick = true;
+    } else {
+      ln.deferred_at = 0.0;
+    }
     Slot &s = *ln.slots[ln.next];
     if (s.used) {
       if (hipEventQuery(s.ev) == hipErrorNotReady) continue;  // every slot of the lane is in flight
       hip_check(hipEventSynchronize(s.ev), "slot wait");
     }
+    if (deferred_tick) ln.deferred_at = now;  // one tick now, the next after another kDeferTickS
     ln.next = (ln.next + 1) % ln.slots.size();
     hipStream_t st = ln.st;
     // sub-step r: the r-th next pass of every job that advances that far

@@ -16,7 +16,10 @@
 // passes: a job of N samples advances by about 2^26 / N passes (1 to 64), so
 // small fields are not paced by the launches of large ones, and the passes of
 // all jobs of a lane that need the same kernel (same plan, pass kind, fiber
-// length and span) merge into one launch (blockIdx.y = job).  A job admitted
+// length and span) merge into one launch (blockIdx.y = job).  Lanes of the
+// smallest grid with passes to issue go first: a lane of a larger grid waits
+// while one of a smaller grid has work, but for one tick a second, so short
+// requests finish without sharing the device with long ones.  A job admitted
 // while others are mid-way simply joins the next tick's launches.  Every job
 // writes its own detector rows; a request's result is the weighted sum over
 // its jobs taken on the host in energy order, so it does not depend on which
