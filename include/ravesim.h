@@ -347,8 +347,9 @@ typedef struct rs_request {
  * that cannot fit with nothing else resident fails its request (poll reports
  * state 3 and returns RS_ERR_DEVICE with an RS_ERR_NOMEM message).  max_jobs
  * bounds the number of jobs resident at once (<= 0: no bound).  Grid shapes
- * are spread over up to four streams, so requests of different N run
- * concurrently. */
+ * are spread over up to four streams; the stream of the smallest grid with
+ * work goes first, and streams of larger grids get one tick a second while
+ * it has work, so short requests are not held up by long ones. */
 int rs_server_create(rs_ctx *ctx, double reserve_bytes, int max_jobs, rs_server **out);
 /* Queue a request (all inputs copied); returns immediately. */
 int rs_server_submit(rs_server *srv, const rs_request *req, int64_t *id);
