@@ -63,6 +63,14 @@ __device__ __forceinline__ typename C2T<S>::t gload(gptr<nvec2<S>> p, uint32_t o
   return o;
 }
 template <typename S>
+__device__ __forceinline__ typename C2T<S>::t gload_nt(gptr<nvec2<S>> p, uint32_t off_bytes) {
+  const nvec2<S> r = __builtin_nontemporal_load((gptr<nvec2<S>>)((gptr<char>)p + off_bytes));
+  typename C2T<S>::t o;
+  o.x = r.x;
+  o.y = r.y;
+  return o;
+}
+template <typename S>
 __device__ __forceinline__ void gstore(gptr<nvec2<S>> p, uint32_t off_bytes, typename C2T<S>::t v) {
   nvec2<S> r;
   r.x = v.x;

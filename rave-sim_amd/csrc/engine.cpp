@@ -771,12 +771,16 @@ template <typename S> struct BatchRun {
       }
     }
 
+    // resident fields larger than the last-level cache stream from HBM
+    const size_t resident = (size_t)lanes * width * fbytes * per_member;
+    const bool streaming = rc.mode == 0 && resident > kMallBytes;
     for (size_t g0 = 0, gi = 0; g0 < members.size(); g0 += group, ++gi) {
       const size_t g1 = std::min(members.size(), g0 + group);
       const int lane = (int)(gi % lanes);
       auto ex = std::make_unique<Exec<S>>(ctx, rc, *plan);
       ex->st = ctx.lane_stream(lane);
       ex->spans = spans;
+      ex->stream_top = streaming;
       for (size_t i = g0; i < g1; ++i) {
         ex->mem.push_back(&members[i]);
         ex->field.push_back(rc.fields ? rc.fields[i] : buf.field[lane][i - g0].p);
@@ -793,8 +797,7 @@ template <typename S> struct BatchRun {
     // bits (N/8 bytes per mask).  Only when the resident fields overflow the
     // last-level cache anyway: when they fit, the extra stream evicts them
     // and costs more than the decisions it saves.
-    const size_t resident = (size_t)lanes * width * fbytes * per_member;
-    if (rc.mode == 0 && resident > kMallBytes)
+    if (streaming)
       for (auto &ex : groups) masks.collect(*ex);
     masks.fill(rc.N, ctx.stream);
   }

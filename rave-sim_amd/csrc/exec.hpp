@@ -172,6 +172,9 @@ template <typename S> struct Exec {
   // as `spans` consecutive sub-ranges of the fields, each small enough to stay
   // in the last-level cache from its first pass to its last
   int spans = 1;
+  // fields larger than the last-level cache: the top column passes read them
+  // with non-temporal loads (Geom::nt)
+  bool stream_top = false;
 
   Exec(Context &c, const RunConfig &r, Plan &p) : ctx(c), rc(r), plan(p), st(c.stream) {}
 
@@ -188,6 +191,7 @@ template <typename S> struct Exec {
     l.row = row;
     l.xv = xop_variant(a.data(), (int)a.size(), g.top != 0);
     l.geom = g;
+    l.geom.nt = stream_top && !row && g.top ? 1 : 0;
     l.nf = (int)a.size();
     l.arg_off = push_args(a);
     launches.push_back(l);

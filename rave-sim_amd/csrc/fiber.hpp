@@ -58,7 +58,8 @@ template <int L, int C, bool ROW> struct TileMap {
   }
 };
 
-template <typename S, int L, int C, bool ROW, int XV>
+// NT: non-temporal field loads (top passes over fields that stream from HBM)
+template <typename S, int L, int C, bool ROW, int XV, bool NT = false>
 __global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
@@ -140,8 +141,19 @@ __global__ void __launch_bounds__(C *(L / 16))
   T2 v[16];
   if (!gen) {
     const gptr<nvec2<S>> p0 = data + nb;
+    // top passes over fields that stream from HBM read them with the
+    // non-temporal hint: each sample is read once and overwritten in place
+    // (MI355X, 2^26 complex64 top pass: 218 -> 212 us per launch; on fields
+    // that the last-level cache holds the hint costs, 181 -> 231 us).  A
+    // compile-time choice: a run-time branch between two loads of the same
+    // address lets the compiler merge them and drop the hint.
+    if constexpr (NT) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb);
+      for (int m = 0; m < 16; ++m) v[m] = gload_nt<S>(p0 + m * mst, lb);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb);
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
@@ -332,6 +344,15 @@ static hipError_t launch_col_c(const FieldArgs *args, int nf, const Geom &g, hip
   int64_t nblk;
   if (!span_tiles(g, (int64_t)L * C, gs, nblk)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk, (unsigned)nf);
+  // non-temporal variant: top passes (an x-space operation inside) of up to
+  // 512, the lengths a plan's top group takes on fields beyond the cache
+  if constexpr (XV != XV_NONE && L <= 512) {
+    if (g.nt) {
+      hipExtLaunchKernelGGL((fiber_pass<S, L, C, false, XV, true>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0,
+                            args, gs);
+      return hipGetLastError();
+    }
+  }
   hipExtLaunchKernelGGL((fiber_pass<S, L, C, false, XV>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0, args, gs);
   return hipGetLastError();
 }

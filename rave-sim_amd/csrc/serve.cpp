@@ -284,6 +284,8 @@ template <typename S> void Server::compile_job(Job &j) {
   // for batched launches when there is one
   Plan &plan = ctx_.plan(c.N, c.f64, -1, -1, kServeBatchHint);
   exec::Exec<S> ex(ctx_, rc, plan);
+  // a field larger than the last-level cache streams from HBM in its top passes
+  ex.stream_top = (size_t)j.N * 2 * sizeof(S) > kMallBytes;
   ex.mem = {&j.m};
   ex.field = {j.field};
   if (j.snap) ex.snap = {j.snap};

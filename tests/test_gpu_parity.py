@@ -362,6 +362,29 @@ def test_graph_capture_matches_eager(dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
+def test_streaming_groups_give_identical_rows(dtype):
+    """Three 2^24 members run one at a time (a group inside the last-level
+    cache: inline grating test, ordinary loads) and as one group of three
+    (beyond the cache: cached grating bits and the non-temporal top pass):
+    the same samples go through the same arithmetic, and the detector sums
+    run in the same member order, so the rows are identical."""
+    import ravesim
+    from ravesim.core import BatchMember, run_batch
+    N, dx = 2 ** 24, 2e-9
+    c = ravesim.Context(0)
+    c.set_split(N, dtype, 12, 6)  # row 4096 | top 64 | middle 64
+    g = ravesim.Grating(pitch=4.2e-6, dc=(0.5, 0.5), z_start=0.0, thickness=20e-6, nr_steps=4,
+                        db_a=complex(7.7e-7, 3.5e-10))
+    ms = [BatchMember(wl=ravesim.energy_to_wavelength(e), gratings=[g], cutoff_angles=[0.3] * 2, weight=0.5 + k,
+                      plane_wave=True) for k, e in enumerate((30e3, 40e3, 50e3))]
+    kw = dict(det_size=5e-4, det_px=1e-6, det_py=1.0, z_detector=0.05, dtype=dtype)
+    one = run_batch(c, N, dx, ms, max_resident=1, **kw)
+    three = run_batch(c, N, dx, ms, max_resident=3, **kw)
+    c.close()
+    assert np.array_equal(one, three)
+
+
+@pytest.mark.parametrize("dtype", [np.complex64, np.complex128])
 def test_spans_give_identical_rows(dtype):
     """The middle, row and middle passes of each propagation run over 1..64
     consecutive sub-ranges of the fields (a 2^26 complex64 field runs over two
