@@ -58,15 +58,33 @@ template <int L, int C, bool ROW> struct TileMap {
   }
 };
 
+// Stride of the fiber slabs in LDS (complex entries): at least L + L/16 (one
+// pad entry per 16).  LDS serves a 64-bit access 16 lanes per cycle (32
+// banks of 4 bytes; 8 lanes for complex128): G = 16 (complex64) or 8
+// (complex128) lanes share a cycle.  Lanes of a column tile take consecutive
+// columns, so a group holds min(C, G) fibers at G / C consecutive positions
+// each: with C >= G an odd stride spreads the fibers over the banks; with
+// fewer fibers the stride is G / C mod G, which puts fiber c at c G / C and
+// its G / C positions after it, all G distinct (MI355X, 2^24 wide top pass
+// of 2048 x 8: 378 bank-conflict cycles per wave with the odd stride, as a
+// model of these groups predicts).  Rows and single fibers keep the odd
+// stride.
+template <typename S> constexpr int fiber_stride(int L, int C, bool row) {
+  const int base = L + L / 16;
+  constexpr int G = sizeof(S) == 4 ? 16 : 8;
+  if (row || C == 1 || C >= G) return base | 1;
+  int f = base;
+  while (f % G != G / C) ++f;
+  return f;
+}
+
 // NT: non-temporal field loads (top passes over fields that stream from HBM)
 template <typename S, int L, int C, bool ROW, int XV, bool NT = false>
 __global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
   constexpr int TPF = L / 16;
-  // fiber slab stride: odd, so that lanes working on neighbouring fibers at
-  // the same offset (column passes) hit different LDS banks
-  constexpr int FIB = (L + L / 16) | 1;
+  constexpr int FIB = fiber_stride<S>(L, C, ROW);
   __shared__ T2 lds[C * FIB];
   // complex64 row passes: the spectrum phase table, gathered at random
   // indices, is served from LDS
