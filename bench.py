@@ -85,9 +85,10 @@ def ensure_tuned(ctx, N, dtype, members=1, retune=False):
     return {**ctx.tuned(N, dtype, members), "tuned_now_s": round(dt, 3)}
 
 
-def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
+def batch_vs_plain(ravesim, ctx, lgN=24, reps=2, plain_leg=True):
     """Batched fused complex64 form vs the plain complex128 form run one
-    simulation at a time, on the polychromatic batch (sims/hour each)."""
+    simulation at a time, on the polychromatic batch (sims/hour each);
+    plain_leg False: the fused form only (for counter runs)."""
     import numpy as np
     from ravesim.core import run_batch
     N = 2 ** lgN
@@ -110,6 +111,11 @@ def batch_vs_plain(ravesim, ctx, lgN=24, reps=2):
         fused_graph = run_batch(ctx, N, 2e-9, ms, dtype=np.complex64, use_graph=True, **kw)
         graph_s.append(time.perf_counter() - t0)
     tf = sum(graph_s) / reps
+    if not plain_leg:
+        return {"workload": f"64 energies x N=2^{lgN}, {steps} steps each, weighted detector sum",
+                "fused_c64_batched_sims_per_hour": round(len(ms) / tf * 3600, 1), "split": tuned,
+                "fused_graph_rep_s": [round(x, 4) for x in graph_s],
+                "fused_eager_rep_s": [round(x, 4) for x in rep_s], "plain": "skipped (--no-plain)"}
     # the baseline at the faster of its default split and the tuned one
     plain_runs = {}
     for label in ("default", "tuned"):
@@ -444,6 +450,8 @@ def main():
     ap.add_argument("--tune-save", default="", help="write the context's tuning table here at the end")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_talbot_lau_n26.json"))
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain comparisons")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batch comparison: run the fused form only (no plain complex128 leg)")
     ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
     ap.add_argument("--no-simdir", action="store_true", help="skip the sim-dir (PCIe-inclusive) run")
@@ -543,9 +551,9 @@ def main():
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
     progress(f"timed chain: {sims_per_hour:.0f} sims/h; top pass {achieved:.0f} GB/s, copy {copy_gbs:.0f} GB/s")
-    batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx)
+    batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx, plain_leg=not args.no_plain)
     if batch:
-        progress(f"polychromatic batch: {batch['speedup']}x")
+        progress(f"polychromatic batch: {batch.get('speedup', 'fused only')}x")
     simdir = None
     if not (args.no_simdir or args.plain):
         try:
