@@ -168,6 +168,60 @@ template <bool INV, typename T2> __device__ __forceinline__ void dft16(T2 *x) {
   }
 }
 
+// fused multiply-add in the field precision
+__device__ __forceinline__ float fmaS(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaS(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// a + x * w (x * conj(w) for the inverse) as four fused multiply-adds
+template <bool INV, typename T2> __device__ __forceinline__ T2 add_tw(T2 a, T2 x, T2 w) {
+  T2 r;
+  if (!INV) {
+    r.x = fmaS(x.x, w.x, fmaS(-x.y, w.y, a.x));
+    r.y = fmaS(x.x, w.y, fmaS(x.y, w.x, a.y));
+  } else {
+    r.x = fmaS(x.x, w.x, fmaS(x.y, w.y, a.x));
+    r.y = fmaS(x.y, w.x, fmaS(-x.x, w.y, a.y));
+  }
+  return r;
+}
+// 2 a - t (t = a + y: a - y from the sum, one rounding)
+template <typename T2> __device__ __forceinline__ T2 twice_minus(T2 a, T2 t) {
+  using S = decltype(a.x);
+  T2 r;
+  r.x = fmaS((S)2, a.x, -t.x);
+  r.y = fmaS((S)2, a.y, -t.y);
+  return r;
+}
+
+// dft16 of x[r] * w[r] (w[0] = 1): the stage twiddles of x[p + 8] and
+// x[p + 12] are fused into the first radix-4 layer, a + c w = four fused
+// multiply-adds and a - c w = 2 a - (a + c w) two more, against a complex
+// product and two sums (16 VALU instructions fewer per stage)
+template <bool INV, typename T2> __device__ __forceinline__ void dft16_tw(T2 *x, const T2 *w) {
+  T2 g[4][4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const T2 a = p ? (INV ? cmulc(x[p], w[p]) : cmul(x[p], w[p])) : x[p];
+    const T2 b = INV ? cmulc(x[p + 4], w[p + 4]) : cmul(x[p + 4], w[p + 4]);
+    const T2 t0 = add_tw<INV>(a, x[p + 8], w[p + 8]);
+    const T2 t1 = twice_minus(a, t0);
+    const T2 t2 = add_tw<INV>(b, x[p + 12], w[p + 12]);
+    const T2 t3 = mul_mi<INV>(twice_minus(b, t2));
+    g[p][0] = cadd(t0, t2); g[p][2] = csub(t0, t2);
+    g[p][1] = cadd(t1, t3); g[p][3] = csub(t1, t3);
+  }
+#pragma unroll
+  for (int p = 1; p < 4; ++p)
+#pragma unroll
+    for (int q = 1; q < 4; ++q) g[p][q] = w16<INV>(g[p][q], p * q);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    T2 h0 = g[0][q], h1 = g[1][q], h2 = g[2][q], h3 = g[3][q];
+    dft4<INV>(h0, h1, h2, h3);
+    x[q] = h0; x[q + 4] = h1; x[q + 8] = h2; x[q + 12] = h3;
+  }
+}
+
 template <int R, bool INV, typename T2> __device__ __forceinline__ void dftR(T2 *x) {
   if constexpr (R == 2) dft2<INV>(x[0], x[1]);
   else if constexpr (R == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
@@ -249,8 +303,10 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
     if constexpr (FIRST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) x[i][r] = v[i + r * NB];
+      dftR<R, INV>(x[i]);
     } else {
       // R = 16 here (the remainder radix only starts a transform): b < L/R
+      static_assert(R == 16, "later stages are radix 16");
       const T2 *src = fib + lds_pad(b);
 #pragma unroll
       for (int r = 0; r < R; ++r) x[i][r] = src[lds_step<LR>(r)];
@@ -269,10 +325,10 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
         const uint32_t e = (uint32_t)(b % NS) * (uint32_t)(L / (NS * R));
         twiddle_powers<T2, R>(twf, e, (uint32_t)L - 1, w);
       }
-#pragma unroll
-      for (int r = 1; r < R; ++r) x[i][r] = INV ? cmulc(x[i][r], w[r]) : cmul(x[i][r], w[r]);
+      w[0].x = 1;
+      w[0].y = 0;
+      dft16_tw<INV>(x[i], w);
     }
-    dftR<R, INV>(x[i]);
   }
   if constexpr (LAST) {
 #pragma unroll
