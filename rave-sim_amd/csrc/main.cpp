@@ -4,8 +4,10 @@
 //   fastwave <sim_dir> -s <source_idx> [--history_dz <f64>] [-h]
 //
 // plus options of this build: --dtype c8|c16 (override config.yaml),
-// --plain (one kernel per operation), --device <n>.
+// --plain (one kernel per operation), --device <n>, --tune <file>|none.
 // Writes <sim_dir>/%08d/detected.npy (float64, (n_steps, n_px)).
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,6 +24,8 @@ static void usage(const char *argv0) {
       "  --dtype c8|c16         Override the precision given in config.yaml\n"
       "  --plain                Run one kernel per operation instead of fused passes\n"
       "  --device <n>           HIP device index (default 0)\n"
+      "  --tune <file>|none     Tuning table to load (default: data/tuned_mi355x.txt beside\n"
+      "                         this executable, when present; entries of other devices are ignored)\n"
       "  -h, --help             print this help screen\n",
       argv0);
 }
@@ -32,6 +36,7 @@ int main(int argc, char **argv) {
   bool have_src = false;
   int dtype = 0, mode = RS_MODE_FUSED, device = 0;
   double history_dz = 0;
+  std::string tune;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto need = [&](const char *what) -> const char * {
@@ -69,6 +74,8 @@ int main(int argc, char **argv) {
       mode = RS_MODE_PLAIN;
     } else if (a == "--device") {
       device = std::atoi(need("device"));
+    } else if (a == "--tune") {
+      tune = need("tune");
     } else if (!a.empty() && a[0] == '-') {
       std::fprintf(stderr, "unknown option %s\n\n", a.c_str());
       usage(argv[0]);
@@ -80,6 +87,19 @@ int main(int argc, char **argv) {
       usage(argv[0]);
       return 1;
     }
+  }
+  if (tune.empty()) {
+    // the table shipped beside the executable (ravesim/data/), if any
+    char exe[4096];
+    const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
+    if (n > 0) {
+      exe[n] = 0;
+      std::string d(exe);
+      d = d.substr(0, d.rfind('/') + 1) + "data/tuned_mi355x.txt";
+      if (access(d.c_str(), R_OK) == 0) tune = d;
+    }
+  } else if (tune == "none") {
+    tune.clear();
   }
   if (!have_src) {
     std::fprintf(stderr, "Source index is required\n\n");
@@ -95,6 +115,14 @@ int main(int argc, char **argv) {
   if (rs_ctx_create(device, &ctx) != RS_OK) {
     std::fprintf(stderr, "uncaught exception in the fast-wave framework: %s\n", rs_last_error());
     return 1;
+  }
+  if (!tune.empty()) {
+    int32_t n = 0;
+    if (rs_tune_load(ctx, tune.c_str(), &n) != RS_OK) {
+      std::fprintf(stderr, "cannot load tuning table %s: %s\n", tune.c_str(), rs_last_error());
+      rs_ctx_destroy(ctx);
+      return 1;
+    }
   }
   rs_run_options o{};
   o.dtype = dtype;

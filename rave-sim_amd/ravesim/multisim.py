@@ -25,11 +25,18 @@ logger = logging.getLogger("big-wave")
 
 _default_ctx: Optional[Context] = None
 
+# the tuning table shipped with the package: digit-group splits measured on an
+# MI355X; rs_tune_load takes only the entries of the context's own device
+DEFAULT_TUNE_TABLE = Path(__file__).resolve().parent / "data" / "tuned_mi355x.txt"
+
 
 def _ctx(device: int = 0) -> Context:
+    """The context of calls that pass none, with the shipped tuning table."""
     global _default_ctx
     if _default_ctx is None:
         _default_ctx = Context(device)
+        if DEFAULT_TUNE_TABLE.exists():
+            _default_ctx.tune_load(DEFAULT_TUNE_TABLE)
     return _default_ctx
 
 

@@ -764,6 +764,46 @@ def test_fastwave_cli_matches_oracle(tmp_path):
     assert hist.shape == h.get_history().shape and rel_l2(hist, h.get_history()) < 1e-10
 
 
+def test_fastwave_cli_tuning_table(tmp_path):
+    """fastwave --tune: a table fixing the case's split (9|7), the shipped
+    default and `none` all give rows that match the oracle; a table that
+    cannot be read is an error (exit 1) with a message."""
+    import subprocess
+    from pathlib import Path
+    import ravesim
+    from ravesim.simdir import write_simdir
+    from simcases import dba_case
+    exe = Path(__file__).resolve().parent.parent / "rave-sim_amd" / "ravesim" / "fastwave"
+    cfg, subs, ang = dba_case(dtype="c16")
+    sd = write_simdir(tmp_path / "sim", cfg, subs, ang)
+    ref = bw.run_simdir(sd, 0, dtype=np.complex128)
+    table = tmp_path / "split.txt"
+    c = ravesim.Context(0)
+    set_plan_tiles(c, table, 2 ** 16, np.complex128, 9, 7, 0, 0)
+    c.close()
+    for extra in (["--tune", str(table)], [], ["--tune", "none"]):
+        r = subprocess.run([str(exe), str(sd), "-s", "0", *extra], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (extra, r.stderr)
+        assert rel_l2(np.load(sd / "00000000" / "detected.npy"), ref) < 1e-10, extra
+    r = subprocess.run([str(exe), str(sd), "-s", "0", "--tune", str(tmp_path / "missing.txt")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 1 and "cannot load tuning table" in r.stderr
+
+
+def test_default_context_loads_the_shipped_table():
+    """Calls that pass no context use one with the shipped MI355X table: it
+    holds the same entry for a 2^26 complex64 simulation as a fresh context
+    that loads the table (none, on a device the table was not measured on)."""
+    import ravesim
+    from ravesim import multisim
+    fresh = ravesim.Context(0)
+    n = fresh.tune_load(multisim.DEFAULT_TUNE_TABLE)
+    expect = fresh.tuned(2 ** 26, np.complex64, 1)
+    fresh.close()
+    assert multisim._ctx().tuned(2 ** 26, np.complex64, 1) == expect
+    assert (expect is not None) == (n > 0)
+
+
 def test_talbot_lau_full_size_precision_form(ctx, tmp_path):
     """The precision form at the benchmark's full size: Talbot-Lau chain at
     N = 2^26 with G2 stepped over 8 positions (16 slices per grating).  Beyond

@@ -296,6 +296,9 @@ def test_fastwave_cli_arguments():
     import subprocess
     r = subprocess.run([str(FASTWAVE), "-h"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "--source_idx" in r.stdout and "--history_dz" in r.stdout
+    assert "--tune" in r.stdout
+    r = subprocess.run([str(FASTWAVE), "x", "-s", "0", "--tune"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "missing value for tune" in r.stderr
     r = subprocess.run([str(FASTWAVE), "/nonexistent"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "Source index is required" in r.stderr
     r = subprocess.run([str(FASTWAVE), "-s", "0"], capture_output=True, text=True, timeout=60)
