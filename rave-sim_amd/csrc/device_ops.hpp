@@ -290,7 +290,9 @@ template <bool WAVE> __device__ __forceinline__ void fiber_sync() {
   }
 }
 
-template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV, bool WAVE>
+// TS: the large stages read W_L^e as entry e * TS of a table of L * TS
+// entries (a sub-fiber of a longer fiber shares that fiber's table)
+template <typename T2, int L, int R, int NS, bool FIRST, bool LAST, bool INV, bool WAVE, int TS = 1>
 __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
                                                const T2 *__restrict__ twf, const T2 *stab) {
   constexpr int TPF = L / 16;
@@ -322,8 +324,8 @@ __device__ __forceinline__ void stockham_stage(T2 (&v)[16], T2 *fib, int q,
 #pragma unroll
         for (int r = 1; r < R; ++r) w[r] = row[r];
       } else {
-        const uint32_t e = (uint32_t)(b % NS) * (uint32_t)(L / (NS * R));
-        twiddle_powers<T2, R>(twf, e, (uint32_t)L - 1, w);
+        const uint32_t e = (uint32_t)(b % NS) * (uint32_t)(L / (NS * R) * TS);
+        twiddle_powers<T2, R>(twf, e, (uint32_t)(L * TS) - 1, w);
       }
       w[0].x = 1;
       w[0].y = 0;
@@ -360,14 +362,84 @@ template <int L, int NS> constexpr int stage_radix() {
   return r;
 }
 
-template <typename T2, int L, int NS, bool INV, bool WAVE = false>
+template <typename T2, int L, int NS, bool INV, bool WAVE = false, int TS = 1>
 __device__ __forceinline__ void stockham(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf,
                                          const T2 *stab) {
   constexpr int R = stage_radix<L, NS>();
   constexpr bool FIRST = NS == 1;
   constexpr bool LAST = NS * R == L;
-  stockham_stage<T2, L, R, NS, FIRST, LAST, INV, WAVE>(v, fib, q, twf, stab);
-  if constexpr (!LAST) stockham<T2, L, NS * R, INV, WAVE>(v, fib, q, twf, stab);
+  stockham_stage<T2, L, R, NS, FIRST, LAST, INV, WAVE, TS>(v, fib, q, twf, stab);
+  if constexpr (!LAST) stockham<T2, L, NS * R, INV, WAVE, TS>(v, fib, q, twf, stab);
+}
+
+// Two-level DFT of a fiber of length L = 16 T (T = L / 16 threads, T >= 16)
+// whose threads span several wavefronts.  With n = n2 + T n1 and
+// k = k1 + 16 k2: a 16-point DFT over each thread's registers (n1), the
+// twiddles W_L^{n2 k1}, one exchange across the fiber (the only workgroup
+// barrier of a direction), then for each k1 a T-point Stockham DFT over n2,
+// held by the G = T / 16 threads q = k1 G .. k1 G + G - 1, which lie in one
+// wavefront (wave-synchronous exchanges in sub-slab k1 of the fiber's slab).
+// Forward: thread q holds x[q + T m] (natural side) before and
+// X[j + T m] after, with j = fiber_spectral(q) = q / G + 16 (q % G) (spectral
+// side); the inverse maps the spectral side back to the natural side.
+// Each thread's cross-fiber reads and writes touch the same 16 LDS words,
+// so an exchange needs no barrier before its writes.
+// The twiddles of the cross step are W_{16 T}^{q k1}: the [k][r] table of the
+// stage at sub-length T when T <= kSmallNS (LDS), else four reads of W_L.
+template <int L> __device__ __forceinline__ int fiber_spectral(int q) {
+  constexpr int G = L / 256;
+  return q / G + 16 * (q % G);
+}
+
+template <typename T2, int L>
+__device__ __forceinline__ void cross_twiddles(int q, const T2 *__restrict__ twf, const T2 *stab, T2 (&w)[16]) {
+  constexpr int T = L / 16;
+  if constexpr (T <= kSmallNS) {
+    const T2 *row = stab + small_tab_offset(L, T) + q * kTwRow;
+#pragma unroll
+    for (int r = 1; r < 16; ++r) w[r] = row[r];
+  } else {
+    twiddle_powers<T2, 16>(twf, (uint32_t)q, (uint32_t)L - 1, w);
+  }
+  w[0].x = 1;
+  w[0].y = 0;
+}
+
+template <typename T2, int L, bool INV>
+__device__ __forceinline__ void fiber_dft_2l(T2 (&v)[16], T2 *fib, int q, const T2 *__restrict__ twf,
+                                             const T2 *stab) {
+  constexpr int T = L / 16, G = T / 16, SUB = T + T / 16;
+  static_assert(G >= 1, "two-level fibers hold at least 256 samples");
+  const int g = q / G, l = q % G;
+  T2 *sub = fib + g * SUB;
+  if constexpr (!INV) {
+    dft16<false>(v);
+    T2 w[16];
+    cross_twiddles<T2, L>(q, twf, stab, w);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], w[k]);
+    // the words written here are the ones this thread read last (inverse
+    // half of a column pass) or untouched (first use)
+    fiber_sync<true>();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) fib[k * SUB + lds_pad(q)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = sub[lds_pad(l + G * m)];
+    stockham<T2, T, 1, false, true, 16>(v, sub, l, twf, stab);
+  } else {
+    stockham<T2, T, 1, true, true, 16>(v, sub, l, twf, stab);
+    // sub-slab g is this wavefront's own until the barrier
+    fiber_sync<true>();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) sub[lds_pad(l + G * m)] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = fib[k * SUB + lds_pad(q)];
+    T2 w[16];
+    cross_twiddles<T2, L>(q, twf, stab, w);
+    dft16_tw<true>(v, w);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -21,10 +21,15 @@ namespace rs {
 // nb + lane + m * mst (m < 16) of fiber c (column c0 + c for column passes),
 // fiber positions q + m * L/16.  nb and mst are uniform over the workgroup,
 // lane * sizeof(complex) < 2^31 for N <= 2^30.
+// lane_at(j): the lane offset of fiber position j of this thread's fiber.
 template <int L, int C, bool ROW> struct TileMap {
   int c, q;
   int64_t nb, c0, mst;
   uint32_t lane;
+  int64_t cst = 0;  // column stride (column passes)
+  __device__ __forceinline__ uint32_t lane_at(int j) const {
+    return ROW ? (uint32_t)(c * L + j) : (uint32_t)(c + (int64_t)j * cst);
+  }
   __device__ __forceinline__ TileMap(const Geom &g, int64_t tile, int t) {
     constexpr int TPF = L / 16;
     c = ROW ? t / TPF : t % C;
@@ -36,6 +41,7 @@ template <int L, int C, bool ROW> struct TileMap {
       lane = (uint32_t)(c * L + q);
       return;
     }
+    cst = g.S;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; when a
     // row segment of C columns is shorter than a 128 B line, the `segs`
     // tiles that share one line are given ids on the same XCD so that its L2
@@ -106,6 +112,15 @@ __global__ void __launch_bounds__(C *(L / 16))
   // consecutive columns, for whole row segments per load; wave-local columns,
   // 64 / TPF per wave, made the 2^26 top pass slower, 216 -> 347 us at 64)
   constexpr bool WAVE = ROW ? TPF <= 64 : C * TPF <= 64;
+  // fibers over several wavefronts and at least 256 long take the two-level
+  // DFT (fiber_dft_2l): one workgroup barrier per direction, the rest of the
+  // exchanges wave-synchronous.  Its spectral side (the frequency-ordered
+  // samples: the outputs of the forward DFT, the inputs of the inverse) puts
+  // fiber position j = fiber_spectral(q) + m L/16 in v[m]; the natural side
+  // keeps j = q + m L/16.  The memory layout is the same either way.
+  constexpr bool TL = !WAVE && TPF >= 16;
+  // each sub-fiber's L/256 threads (times C columns) lie in one wavefront
+  static_assert(!TL || (ROW ? TPF / 16 : (TPF / 16) * C) <= 64, "two-level groups span wavefronts");
 
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
@@ -118,6 +133,15 @@ __global__ void __launch_bounds__(C *(L / 16))
   const int64_t mst = tm.mst;
   const uint32_t lane = tm.lane;
   const uint32_t lb = lane * (uint32_t)sizeof(T2);
+  // position base of the spectral side (twiddles, spectrum bins) and its
+  // lane offset in bytes
+  const int qs = TL ? fiber_spectral<L>(q) : q;
+  const uint32_t lbs = TL ? tm.lane_at(qs) * (uint32_t)sizeof(T2) : lb;
+  // loads read the spectral side when the pass starts with an inverse
+  // (column passes) or has no forward (row passes); stores likewise
+  const bool spec_in = ROW ? !A.do_fwd : A.do_inv;
+  const bool spec_out = ROW ? !A.do_inv : A.do_fwd;
+  const uint32_t lb_in = spec_in ? lbs : lb, lb_out = spec_out ? lbs : lb;
   T2 *fib = lds + c * FIB;
 
   const gptr<nvec2<S>> data = global_ptr<S>(A.data);
@@ -149,7 +173,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       for (int i = t; i < kCisTab; i += C * TPF) ktab[i] = gt[i];
     }
     if constexpr (CPOW) build_col_powers<false, T2, L, C>(g, c0, cpow, t);
-    if (CPOW || kt || (WAVE && NSTAB > 0)) __syncthreads();
+    if (CPOW || kt || ((WAVE || TL) && NSTAB > 0)) __syncthreads();
   }
   // cached grating decisions of this thread, loaded with the field (their
   // first use comes after the inverse DFT's barriers)
@@ -167,17 +191,17 @@ __global__ void __launch_bounds__(C *(L / 16))
     // address lets the compiler merge them and drop the hint.
     if constexpr (NT) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = gload_nt<S>(p0 + m * mst, lb);
+      for (int m = 0; m < 16; ++m) v[m] = gload_nt<S>(p0 + m * mst, lb_in);
     } else {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb);
+      for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb_in);
     }
   } else {
 #pragma unroll
     for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
   }
   T2 wq;
-  if constexpr (CPOW) wq = col_base_twiddle<T2, L>(g, q, c0 + c);
+  if constexpr (CPOW) wq = col_base_twiddle<T2, L>(g, qs, c0 + c);
   if constexpr (EARLY) {
     LdsStage<T2, NSTAB, C * TPF> st;
     st.load(twf + L, t);
@@ -186,7 +210,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     if constexpr (CPOW) build_col_powers<true, T2, L, C>(g, c0, cpow, t);
     st.store(stab, t);
     if (kt) sk.store(ktab, t);
-    if (CPOW || kt || (WAVE && NSTAB > 0)) __syncthreads();
+    if (CPOW || kt || ((WAVE || TL) && NSTAB > 0)) __syncthreads();
   }
   {
     const int64_t col = c0 + c;
@@ -200,15 +224,16 @@ __global__ void __launch_bounds__(C *(L / 16))
     if (!ROW && A.do_inv) {
       if constexpr (KEEP_TW) {
         if constexpr (CPOW) col_twiddles<T2, C>(wq, c, cpow, tw);
-        else inter_twiddles<T2, L>(g, q, col, tw);
+        else inter_twiddles<T2, L>(g, qs, col, tw);
         have_tw = true;
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = cmulc(v[m], tw[m]);
       } else {
         if constexpr (CPOW) apply_col_twiddles<true, T2, C>(wq, c, cpow, v);
-        else apply_inter_twiddles<true, T2, L>(g, q, col, v);
+        else apply_inter_twiddles<true, T2, L>(g, qs, col, v);
       }
-      stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
+      if constexpr (TL) fiber_dft_2l<T2, L, true>(v, fib, q, twf, stab);
+      else stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
     }
 
     if constexpr (XV == XV_MASK) {
@@ -241,7 +266,8 @@ __global__ void __launch_bounds__(C *(L / 16))
     }
 
     if (A.do_fwd) {
-      stockham<T2, L, 1, false, WAVE>(v, fib, q, twf, stab);
+      if constexpr (TL) fiber_dft_2l<T2, L, false>(v, fib, q, twf, stab);
+      else stockham<T2, L, 1, false, WAVE>(v, fib, q, twf, stab);
       if (!ROW) {
         if (have_tw) {
 #pragma unroll
@@ -249,7 +275,7 @@ __global__ void __launch_bounds__(C *(L / 16))
         } else if constexpr (CPOW) {
           apply_col_twiddles<false, T2, C>(wq, c, cpow, v);
         } else {
-          apply_inter_twiddles<false, T2, L>(g, q, col, v);
+          apply_inter_twiddles<false, T2, L>(g, qs, col, v);
         }
       }
     }
@@ -259,7 +285,7 @@ __global__ void __launch_bounds__(C *(L / 16))
       const int64_t kstep = g.N >> g.lgR;
       if constexpr (sizeof(S) == 4) {
         const KopF32 kp = kop_f32_setup(A.k, ktab);
-        const int k0 = (int)(kb + kstep * q);  // < N/16
+        const int k0 = (int)(kb + kstep * qs);  // < N/16
         const int kst = (int)(kstep * TPF);    // = N/16
         const KopQuad kq = kop_f32_quad(kp, k0, g.N);
         if (kp.rlen != 0) {
@@ -272,18 +298,21 @@ __global__ void __launch_bounds__(C *(L / 16))
       } else {
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
-          const int64_t k = kb + kstep * (q + m * TPF);
+          const int64_t k = kb + kstep * (qs + m * TPF);
           v[m] = apply_kop(A.k, k, g.N, v[m]);
         }
       }
     }
 
-    if (ROW && A.do_inv) stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
+    if (ROW && A.do_inv) {
+      if constexpr (TL) fiber_dft_2l<T2, L, true>(v, fib, q, twf, stab);
+      else stockham<T2, L, 1, true, WAVE>(v, fib, q, twf, stab);
+    }
 
     {
       const gptr<nvec2<S>> p0 = data + nb;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) gstore<S>(p0 + m * mst, lb, v[m]);
+      for (int m = 0; m < 16; ++m) gstore<S>(p0 + m * mst, lb_out, v[m]);
     }
 
   }
