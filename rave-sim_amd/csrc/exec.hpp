@@ -267,8 +267,10 @@ template <typename S> struct Exec {
       for (int i = 0; i < K; ++i) { a[i].do_fwd = fwd; a[i].do_inv = inv; }
       return a;
     };
+    Geom row = plan.geom_row();
+    row.one_dir = 1;
     if (plan.ngroups == 1) {
-      fiber(plan.lgR, true, plan.geom_row(), mk(!inverse, inverse));
+      fiber(plan.lgR, true, row, mk(!inverse, inverse));
       return;
     }
     Geom top = plan.geom_top();
@@ -276,9 +278,9 @@ template <typename S> struct Exec {
     if (!inverse) {
       fiber(plan.lgA, false, top, mk(true, false));
       if (plan.ngroups == 3) fiber(plan.lgB, false, plan.geom_mid(), mk(true, false));
-      fiber(plan.lgR, true, plan.geom_row(), mk(true, false));
+      fiber(plan.lgR, true, row, mk(true, false));
     } else {
-      fiber(plan.lgR, true, plan.geom_row(), mk(false, true));
+      fiber(plan.lgR, true, row, mk(false, true));
       if (plan.ngroups == 3) fiber(plan.lgB, false, plan.geom_mid(), mk(false, true));
       fiber(plan.lgA, false, top, mk(false, true));
     }

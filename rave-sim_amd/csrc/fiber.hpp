@@ -85,7 +85,8 @@ template <typename S> constexpr int fiber_stride(int L, int C, bool row) {
 }
 
 // NT: non-temporal field loads (top passes over fields that stream from HBM)
-template <typename S, int L, int C, bool ROW, int XV, bool NT = false>
+// SD: a row pass that runs one direction only (the plain form's FFT passes)
+template <typename S, int L, int C, bool ROW, int XV, bool NT = false, bool SD = false>
 __global__ void __launch_bounds__(C *(L / 16))
     fiber_pass(const FieldArgs *__restrict__ args, Geom g) {
   using T2 = typename C2T<S>::t;
@@ -118,7 +119,11 @@ __global__ void __launch_bounds__(C *(L / 16))
   // samples: the outputs of the forward DFT, the inputs of the inverse) puts
   // fiber position j = fiber_spectral(q) + m L/16 in v[m]; the natural side
   // keeps j = q + m L/16.  The memory layout is the same either way.
-  constexpr bool TL = !WAVE && TPF >= 16;
+  // Row passes of one direction keep the Stockham form: their spectral side
+  // is in memory, where the two-level order would spread a wavefront's
+  // accesses 16 samples apart (MI355X, complex128 4096-long rows at 2^24:
+  // 82 against 97 us per pass).
+  constexpr bool TL = !WAVE && TPF >= 16 && !SD;
   // each sub-fiber's L/256 threads (times C columns) lie in one wavefront
   static_assert(!TL || (ROW ? TPF / 16 : (TPF / 16) * C) <= 64, "two-level groups span wavefronts");
 
@@ -413,9 +418,9 @@ static hipError_t launch_col(const FieldArgs *args, int nf, const Geom &g, hipSt
 // rows per workgroup: full waves and 256 threads (complex64) / 128 threads
 // (complex128) for rows up to 4096 / 2048 samples, one row beyond; fields
 // with fewer rows than that (single-group fields) use one row per workgroup
-template <typename S, int L, int XV>
-static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
-                             hipEvent_t t1) {
+template <typename S, int L, int XV, bool SD>
+static hipError_t launch_row_sd(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                                hipEvent_t t1) {
   constexpr int SPAN = sizeof(S) == 8 ? 2048 : 4096;
   constexpr int C = L >= SPAN ? 1 : SPAN / L;
   const int64_t rows = g.N / L;
@@ -423,13 +428,25 @@ static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipSt
   int64_t nblk;
   if (rows % C == 0 && span_tiles(g, (int64_t)L * C, gs, nblk)) {
     dim3 grid((unsigned)nblk, (unsigned)nf);
-    hipExtLaunchKernelGGL((fiber_pass<S, L, C, true, XV>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0, args, gs);
+    hipExtLaunchKernelGGL((fiber_pass<S, L, C, true, XV, false, SD>), grid, dim3(C * (L / 16)), 0, st, t0, t1, 0,
+                          args, gs);
   } else {
     if (!span_tiles(g, (int64_t)L, gs, nblk)) return hipErrorInvalidValue;
     dim3 grid((unsigned)nblk, (unsigned)nf);
-    hipExtLaunchKernelGGL((fiber_pass<S, L, 1, true, XV>), grid, dim3(L / 16), 0, st, t0, t1, 0, args, gs);
+    hipExtLaunchKernelGGL((fiber_pass<S, L, 1, true, XV, false, SD>), grid, dim3(L / 16), 0, st, t0, t1, 0, args,
+                          gs);
   }
   return hipGetLastError();
+}
+
+// rows of one direction (Geom::one_dir) that the two-level form would take
+// run the Stockham variant
+template <typename S, int L, int XV>
+static hipError_t launch_row(const FieldArgs *args, int nf, const Geom &g, hipStream_t st, hipEvent_t t0,
+                             hipEvent_t t1) {
+  if constexpr (L >= 2048 && XV == XV_NONE)
+    if (g.one_dir) return launch_row_sd<S, L, XV, true>(args, nf, g, st, t0, t1);
+  return launch_row_sd<S, L, XV, false>(args, nf, g, st, t0, t1);
 }
 
 template <typename S, int XV>

@@ -111,6 +111,7 @@ struct Geom {
   int32_t xcd_segs;  // column passes: workgroups sharing one 128 B line (1 = none)
   int32_t tile;      // column passes: tile width variant (col_fibers), 0 = default
   int32_t nt;        // top column passes of up to 512: non-temporal field loads (fields streaming from HBM)
+  int32_t one_dir;   // row passes: every field runs one direction only (plain-form FFT passes)
   const void *tw_hi;
   const void *tw_lo;
   const void *tw_fib;  // W_L^j, j < L, then the small-stage tables (rs_kernels.hpp)
