@@ -157,9 +157,11 @@ __global__ void __launch_bounds__(C *(L / 16))
 
   const int64_t nb = tm.nb, c0 = tm.c0;
   // LDS tables: the twiddles of the small Stockham stages (first read after
-  // the first exchange, so a workgroup barrier covers them only where the
-  // exchanges are wave-synchronous), the spectrum phase table and the
-  // column powers.
+  // the first exchange of a Stockham fiber over several wavefronts, so a
+  // workgroup barrier of their own is needed only where the exchanges are
+  // wave-synchronous or the fiber is two-level, whose cross twiddles and
+  // sub-fiber stages can read them before its one barrier), the spectrum
+  // phase table and the column powers.
   // complex64 passes issue the field loads first and then every table read
   // before the first LDS write; the table reads wait for the field loads
   // with their own, so such a pass pays one memory latency before its first
