@@ -388,6 +388,32 @@ def fringe_phase_error(rows, ref):
     return float(np.max(np.abs(np.angle(a[ok] * np.conj(b[ok])))))
 
 
+def fringe_bin_phase_error(row, ref):
+    """Fringe-phase difference (rad) of one detector row: the phase of the
+    row's DFT at the bin where the reference's spectrum peaks (DC excluded),
+    against the reference's phase at that bin."""
+    import numpy as np
+    a = np.fft.rfft(row - row.mean())
+    b = np.fft.rfft(ref - ref.mean())
+    k = 1 + int(np.argmax(np.abs(b[1:])))
+    return float(abs(np.angle(a[k] * np.conj(b[k])))), k
+
+
+def chain_precision(ravesim, ctx, N, members, kw, out):
+    """The timed chain's complex64 + FP64-phase detector row against the
+    same simulation in the plain complex128 form (the oracle's precision;
+    the oracle itself is checked against that form at smaller N)."""
+    import numpy as np
+    from ravesim.core import run_batch
+    t0 = time.perf_counter()
+    ref = run_batch(ctx, N, 2e-9, members, **{**kw, "dtype": np.complex128, "plain": True})
+    ctx.synchronize()
+    dphi, k = fringe_bin_phase_error(out[0], ref[0])
+    return {"reference": "plain complex128 form, same simulation",
+            "rel_l2": float(np.linalg.norm(out - ref) / np.linalg.norm(ref)),
+            "fringe_phase_err_rad": dphi, "fringe_bin": k, "reference_s": round(time.perf_counter() - t0, 3)}
+
+
 def cpu_baseline(N, budget_s=25.0, workers=1):
     """Oracle (complex128 numpy/scipy restatement, `workers` scipy.fft
     threads; big-wave uses the default single one) on whole grating slices of
@@ -454,6 +480,8 @@ def main():
                     help="batch comparison: run the fused form only (no plain complex128 leg)")
     ap.add_argument("--mix-sims", type=int, default=256, help="simulations of the uneven mix (0 = skip)")
     ap.add_argument("--no-rocfft", action="store_true", help="skip the vendor FFT comparison")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the chain's complex128 reference run")
     ap.add_argument("--no-simdir", action="store_true", help="skip the sim-dir (PCIe-inclusive) run")
     ap.add_argument("--serve-qps", type=float, default=0.2,
                     help="serving run arrival rate in requests/s (0 = skip)")
@@ -551,6 +579,11 @@ def main():
                                "GBs": round(v[2] / max(v[1], 1e-12) / 1e6, 1)} for k, v in stats.items() if v[0]}}
 
     progress(f"timed chain: {sims_per_hour:.0f} sims/h; top pass {achieved:.0f} GB/s, copy {copy_gbs:.0f} GB/s")
+    precision = None
+    if not args.plain and args.dtype == "c8" and not args.no_precision:
+        precision = chain_precision(ravesim, ctx, N, members, kw, out)
+        progress(f"chain vs plain complex128: rel-L2 {precision['rel_l2']:.2e}, "
+                 f"fringe phase {precision['fringe_phase_err_rad']:.2e} rad")
     batch = None if (args.no_batch or args.plain) else batch_vs_plain(ravesim, ctx, plain_leg=not args.no_plain)
     if batch:
         progress(f"polychromatic batch: {batch.get('speedup', 'fused only')}x")
@@ -606,6 +639,7 @@ def main():
         "gsample_steps_per_s": round(gss, 3),
         "hbm_frac_whole_sim": round(gss * 1e9 * (32 if args.dtype == "c8" else 64) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
+        "precision": precision,
         "cpu_baseline": cpu,
         "batched_vs_plain": batch,
         "uneven_mix": mix,
