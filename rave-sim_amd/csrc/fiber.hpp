@@ -46,18 +46,21 @@ template <int L, int C, bool ROW> struct TileMap {
     // row segment of C columns is shorter than a 128 B line, the `segs`
     // tiles that share one line are given ids on the same XCD so that its L2
     // merges their partial-line reads and writes
-    const int64_t ntiles = g.N / ((int64_t)L * C);
-    int64_t bid = tile;
+    // (N, S, C, L and the segment count are powers of two: shifts and masks
+    // of workgroup-uniform values, no 64-bit division in the prologue)
+    constexpr int lgC = __builtin_ctz(C), lgLC = __builtin_ctz(L * C);
+    const uint64_t ntiles = (uint64_t)g.N >> lgLC;
+    uint64_t bid = (uint64_t)tile;
     if (g.xcd_segs > 1) {
-      const int64_t G = 8 * (int64_t)g.xcd_segs;
-      if (bid < (ntiles / G) * G) {
-        const int64_t r = bid % G;
-        bid = (bid - r) + (r % 8) * g.xcd_segs + r / 8;
+      const int lgseg = __builtin_ctz((uint32_t)g.xcd_segs), lgG = 3 + lgseg;
+      if (bid < ((ntiles >> lgG) << lgG)) {
+        const uint64_t r = bid & ((uint64_t(1) << lgG) - 1);
+        bid = (bid - r) + ((r & 7) << lgseg) + (r >> 3);
       }
     }
-    const int64_t cpb = g.S / C;
-    const int64_t hi = bid / cpb;
-    c0 = (bid % cpb) * C;
+    const int lgcpb = __builtin_ctzll((uint64_t)g.S) - lgC;  // column tiles per row of the top group
+    const int64_t hi = (int64_t)(bid >> lgcpb);
+    c0 = (int64_t)((bid & ((uint64_t(1) << lgcpb) - 1)) << lgC);
     nb = hi * (int64_t)L * g.S + c0;
     mst = (int64_t)TPF * g.S;
     lane = (uint32_t)(c + (int64_t)q * g.S);
