@@ -54,6 +54,19 @@ template <typename S> using nvec2 = S __attribute__((ext_vector_type(2)));
 template <typename S> __device__ __forceinline__ gptr<nvec2<S>> global_ptr(void *p) {
   return (gptr<nvec2<S>>)p;
 }
+// A workgroup-uniform base held in scalar registers as an opaque value, so
+// that an access stays (scalar base + 32-bit lane offset) instead of being
+// reassociated into a per-lane 64-bit address (a 64-bit multiply-add and an
+// add per access on the vector unit).
+template <typename S> __device__ __forceinline__ gptr<nvec2<S>> sbase(gptr<nvec2<S>> p) {
+  uint64_t a = (uint64_t)p;
+  asm("" : "+s"(a));
+  return (gptr<nvec2<S>>)a;
+}
+template <bool SB, typename S> __device__ __forceinline__ gptr<nvec2<S>> field_base(gptr<nvec2<S>> p) {
+  if constexpr (SB) return sbase<S>(p);
+  else return p;
+}
 template <typename S>
 __device__ __forceinline__ typename C2T<S>::t gload(gptr<nvec2<S>> p, uint32_t off_bytes) {
   const nvec2<S> r = *(gptr<nvec2<S>>)((gptr<char>)p + off_bytes);

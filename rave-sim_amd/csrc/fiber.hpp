@@ -129,6 +129,14 @@ __global__ void __launch_bounds__(C *(L / 16))
   constexpr bool TL = !WAVE && TPF >= 16 && !SD;
   // each sub-fiber's L/256 threads (times C columns) lie in one wavefront
   static_assert(!TL || (ROW ? TPF / 16 : (TPF / 16) * C) <= 64, "two-level groups span wavefronts");
+  // field accesses from scalar per-m bases (sbase): rows, passes with an
+  // x-space operation, and column tiles of whole 128 B lines.  The wide
+  // tiles of partial lines without one stay on vector addresses, where the
+  // scalar bases measured slower (MI355X, 2^24: complex128 4096 x 2 column
+  // pass 152.7 -> 173.3 us, complex64 4096 x 4 inverse 109.4 -> 117.2 us;
+  // with the grating mask, 4096 x 4 106.3 -> 103.3 us, 2^26 top pass 256 x 32
+  // 213.1 -> 203.5 us).
+  constexpr bool SB = ROW || XV != XV_NONE || C * (int)sizeof(T2) >= 128;
 
   const FieldArgs &A = args[blockIdx.y];
   const T2 *__restrict__ twf = (const T2 *)g.tw_fib;
@@ -189,7 +197,7 @@ __global__ void __launch_bounds__(C *(L / 16))
   // first use comes after the inverse DFT's barriers)
   uint32_t mword = 0;
   if constexpr (XV == XV_MASK && !ROW)
-    if (xop && A.x.kind == XOP_MASK && A.x.mbits) mword = ((const uint16_t *)A.x.mbits)[tile * (C * TPF) + t];
+    if (xop && A.x.kind == XOP_MASK && A.x.mbits) mword = ((gptr<const uint16_t>)A.x.mbits)[tile * (C * TPF) + t];
   T2 v[16];
   if (!gen) {
     const gptr<nvec2<S>> p0 = data + nb;
@@ -201,14 +209,19 @@ __global__ void __launch_bounds__(C *(L / 16))
     // address lets the compiler merge them and drop the hint.
     if constexpr (NT) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = gload_nt<S>(p0 + m * mst, lb_in);
+      for (int m = 0; m < 16; ++m) v[m] = gload_nt<S>(field_base<SB, S>(p0 + m * mst), lb_in);
     } else {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = gload<S>(p0 + m * mst, lb_in);
+      for (int m = 0; m < 16; ++m) v[m] = gload<S>(field_base<SB, S>(p0 + m * mst), lb_in);
     }
   } else {
+    // a generated plane wave: ones.  The empty volatile asm keeps these
+    // moves inside the branch (hoisted, every thread of a reading pass ran
+    // 32 of them before its loads overwrote them).
+    S one = 1, zero = 0;
+    asm volatile("" : "+v"(one), "+v"(zero));
 #pragma unroll
-    for (int m = 0; m < 16; ++m) { v[m].x = 1; v[m].y = 0; }
+    for (int m = 0; m < 16; ++m) { v[m].x = one; v[m].y = zero; }
   }
   T2 wq;
   if constexpr (CPOW) wq = col_base_twiddle<T2, L>(g, qs, c0 + c);
@@ -322,7 +335,7 @@ __global__ void __launch_bounds__(C *(L / 16))
     {
       const gptr<nvec2<S>> p0 = data + nb;
 #pragma unroll
-      for (int m = 0; m < 16; ++m) gstore<S>(p0 + m * mst, lb_out, v[m]);
+      for (int m = 0; m < 16; ++m) gstore<S>(field_base<SB, S>(p0 + m * mst), lb_out, v[m]);
     }
 
   }
